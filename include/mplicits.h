@@ -1,0 +1,106 @@
+/*
+ * mplicits.h - C ABI of the MI355X (gfx950) M-plicits hot path.
+ *
+ * The reference project has no plugin/FFI layer: its boundary is the Python
+ * call surface of i3d (SIREN.forward, create_mesh_multistage, the losses) and
+ * the renderer's CUDA entry points.  This library sits under a Python mirror of
+ * that surface (package `mplicits`) and is bound with ctypes.  Every symbol
+ * below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Plain C types only.  Streams are `hipStream_t` passed as `void*`
+ *    (NULL = the null stream).  Every call is stream-ordered and asynchronous
+ *    unless documented as a host-sync point.
+ *  - Tensor arguments are caller-owned device pointers (torch tensors).
+ *  - Return 0 on success, a negative MP_E* code otherwise; mp_last_error()
+ *    returns the thread-local message of the last failure.  Nothing exits or
+ *    throws across the ABI.
+ *  - A stack handle must not be used from two host threads at once.
+ */
+#ifndef MPLICITS_H
+#define MPLICITS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_ABI_VERSION 1
+
+#define MP_OK 0
+#define MP_EINVAL (-1)
+#define MP_ENOMEM (-2)
+#define MP_EHIP (-3)
+#define MP_EUNSUPPORTED (-4)
+
+/* arithmetic of the hidden-layer GEMMs (first layer, sines, output layer and
+ * the residual sum are always FP32) */
+#define MP_FP32 0
+#define MP_BF16 1
+#define MP_FP16 2
+
+/* sample coordinates of the extraction grid */
+#define MP_GRID_REFERENCE 0 /* i3d/meshing.py:42-63 as torch evaluates it (float32 true division) */
+#define MP_GRID_LATTICE 1   /* the regular lattice x = a*h - 1 */
+
+#define MP_MAX_LEVELS 4
+
+typedef struct mp_stack mp_stack;
+
+/* One SIREN level: i3d/model.py:76-105 (n_in -> hidden[0] -> ... -> n_out),
+ * weights in the renderer's .bin order (renderer/scripts/export_experiment.py:28-43):
+ * `weights` = row-major W_k [out_k][in_k] concatenated for k = 0..n_hidden,
+ * `biases` = b_k concatenated.  The frequency is baked (evaluated with w0 = 1,
+ * i3d/model.py:143-180).  Both are device pointers that the stack borrows. */
+typedef struct mp_level_desc {
+  int n_in;             /* must be 3 */
+  int n_out;            /* must be 1 */
+  int n_hidden;         /* number of sine layers, len(hidden_layer_config) >= 1 */
+  const int* hidden;    /* host array [n_hidden] of widths, each 1..512 */
+  const float* weights; /* device */
+  const float* biases;  /* device */
+} mp_level_desc;
+
+int mp_abi_version(void);
+const char* mp_last_error(void);
+
+/* Build a level stack (coarse level first).  Replaces the per-level
+ * i3d.util.from_pth(...).to(device) (i3d/util.py:79-142) and the renderer's
+ * NeuralImplicit::init (renderer/cuda/gemms.cu:111-134): packs every level into
+ * MFMA fragment order in `precision`.  Allocates device memory. */
+int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, void* stream,
+                    mp_stack** out);
+/* Re-pack from the borrowed weight pointers (after an optimizer step). */
+int mp_stack_update(mp_stack* stack, void* stream);
+void mp_stack_destroy(mp_stack* stack);
+int mp_stack_levels(const mp_stack* stack);
+
+/* Evaluate sum_{l=lvl0}^{lvl1-1} f_l(x) at n points xyz [n][3] into sdf [n].
+ * Replaces SIREN.forward (i3d/model.py:107-129) summed over levels as
+ * create_mesh_multistage does with deltas=[] (i3d/meshing.py:232-233).
+ * If grad != NULL, also writes the input gradient [n][3] of that sum
+ * (i3d/diff_operators.py:259-265; renderer Normals::inference,
+ * renderer/cuda/gemms.cu:263-337). */
+int mp_eval(const mp_stack* stack, const float* xyz, int64_t n, int lvl0, int lvl1, float* sdf,
+            float* grad, void* stream);
+
+/* Band-culled volume evaluation: the sampling loop of create_mesh_multistage
+ * (i3d/meshing.py:203-238) with the same per-chunk culling semantics
+ * (chunk = max_batch, reconstruct.py:128 uses 32768).  n_deltas must be 0
+ * (all levels everywhere) or n_levels-1.  vol is N^3 floats, device.
+ * level_counts (host, n_levels entries, may be NULL) receives the number of
+ * samples each level was evaluated on.  Host-sync point (the band counts are
+ * read back to size the launches).  Allocates device scratch. */
+int mp_extract_volume(mp_stack* stack, int N, int grid_mode, int64_t chunk, const float* deltas,
+                      int n_deltas, float* vol, int64_t* level_counts, void* stream);
+
+/* Sample coordinates of grid index range [first, first+n) into xyz [n][3]
+ * (gen_mc_coordinate_grid, i3d/meshing.py:11-69, columns 0..2). */
+int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPLICITS_H */

@@ -1,0 +1,42 @@
+// Error plumbing shared by every translation unit of libmplicits_hip.so.
+// All C entry points return MP_* status codes; the message of the last failure
+// is kept per host thread and returned by mp_last_error().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/mplicits.h"
+
+namespace mp {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+const char* get_error();
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace mp
+
+// Evaluate a HIP call; on failure record the message and return MP_EHIP.
+#define MP_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t mp_err_ = (call);                                                       \
+    if (mp_err_ != hipSuccess) {                                                       \
+      mp::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(mp_err_), __FILE__, \
+                    __LINE__);                                                         \
+      return MP_EHIP;                                                                  \
+    }                                                                                  \
+  } while (0)
+
+// Check the launch that was just enqueued.
+#define MP_LAUNCHED(name) MP_HIP(hipGetLastError())
+
+#define MP_REQUIRE(cond, code, ...) \
+  do {                              \
+    if (!(cond)) {                  \
+      mp::set_error(__VA_ARGS__);   \
+      return (code);                \
+    }                               \
+  } while (0)

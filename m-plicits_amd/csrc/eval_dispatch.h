@@ -1,0 +1,37 @@
+// Per-level evaluation launches shared by mp_eval, the extraction pipeline and
+// the training forward.
+#pragma once
+
+#include "stack.h"
+
+namespace mp {
+
+enum EvalSrc { SRC_POINTS = 0, SRC_GRID = 1, SRC_LIST = 2 };
+
+// Where the points of one level launch come from and where f goes.
+struct EvalTarget {
+  int src;                       // EvalSrc
+  const float* xyz;              // SRC_POINTS: [n][3]
+  int N, grid_mode;              // SRC_GRID / SRC_LIST: grid size and coordinate mode
+  float h;                       // grid spacing fl32(2 / (N - 1))
+  int64_t first;                 // SRC_GRID: first grid index
+  const uint32_t* list;          // SRC_LIST: grid indices
+  const uint32_t* list_count;    // SRC_LIST: device count (n is a host upper bound)
+  int64_t n;                     // points (or upper bound for SRC_LIST)
+  float* out;                    // SRC_POINTS: out[i]; grid sources: out[grid index]
+  int accumulate;                // out = out + f instead of out = f
+  float band;                    // > 0: append the key of every point with |out| < band
+  uint32_t* cand;
+  uint32_t* cand_count;
+};
+
+int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream);
+
+// Input gradient of sum_{l=lvl0}^{lvl1-1} f_l at n points (overwrites grad).
+int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
+                hipStream_t stream);
+
+// Points handled by one wave of the level kernels.
+int level_tile(const mp_stack* st, int level);
+
+}  // namespace mp

@@ -1,0 +1,47 @@
+// Level-stack handle shared by the translation units of libmplicits_hip.so.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+#include "mlp_core.h"
+
+struct mp_stack {
+  struct Level {
+    int n_in = 3, n_out = 1, n_hidden = 0;
+    std::vector<int> hidden;      // true widths
+    int width = 0;                // padded width
+    const float* weights = nullptr;  // borrowed, renderer .bin order
+    const float* biases = nullptr;   // borrowed
+    void* buf = nullptr;          // packed arrays (one allocation)
+    size_t buf_bytes = 0;
+    mp::LevelDev dev{};
+  };
+  int n_levels = 0;
+  int precision = MP_FP32;
+  int shape = mp::SH_16x16x4_F32;
+  int nc = 1;  // point column blocks per wave
+  Level lv[MP_MAX_LEVELS];
+  void* scratch = nullptr;  // extraction / tracing work lists, grown on demand
+  size_t scratch_bytes = 0;
+};
+
+namespace mp {
+
+// Padded width used by the kernels for a level whose widest layer is `w`.
+inline int padded_width(int w) {
+  if (w <= 64) return 64;
+  if (w <= 128) return 128;
+  if (w <= 256) return 256;
+  if (w <= 512) return 512;
+  return -1;
+}
+
+// Launch-shape selection shared by every kernel family (a tile-shape override
+// from the autotuner is applied at stack creation).
+int default_nc(int shape, int width);
+
+// Ensure st->scratch holds at least `bytes`; returns MP_OK or MP_ENOMEM.
+int ensure_scratch(mp_stack* st, size_t bytes);
+
+}  // namespace mp

@@ -1,0 +1,296 @@
+// Level-stack creation / packing, point evaluation and grid coordinates.
+//
+// mp_stack_create replaces i3d.util.from_pth(...).to(device) for the GPU path
+// (i3d/util.py:79-142) and the renderer's NeuralImplicit::init
+// (renderer/cuda/gemms.cu:111-134): the float32 weights stay where the caller
+// keeps them (a torch state_dict on the device) and are packed into MFMA
+// fragment order by the kernels below.
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "stack.h"
+#include "eval_dispatch.h"
+
+namespace mp {
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+}
+const char* get_error() { return g_error.c_str(); }
+
+int default_nc(int shape, int width) {
+  (void)shape;
+  (void)width;
+  return 1;
+}
+
+int ensure_scratch(mp_stack* st, size_t bytes) {
+  if (st->scratch_bytes >= bytes) return MP_OK;
+  if (st->scratch) {
+    // the previous contents are not needed; work on the stream is ordered
+    // before the free by the device-wide synchronisation hipFree performs
+    (void)hipFree(st->scratch);
+    st->scratch = nullptr;
+    st->scratch_bytes = 0;
+  }
+  const size_t b = (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+  hipError_t e = hipMalloc(&st->scratch, b);
+  if (e != hipSuccess) {
+    set_error("scratch allocation of %zu bytes failed: %s", b, hipGetErrorString(e));
+    return MP_ENOMEM;
+  }
+  st->scratch_bytes = b;
+  return MP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Packing kernels.  Layer k of the level has shape [out_k][in_k]; offsets into
+// the borrowed weight / bias buffers follow the renderer .bin order.
+// ---------------------------------------------------------------------------
+struct PackArgs {
+  const float* W;   // weights of the level
+  const float* B;   // biases of the level
+  int shape;
+  int width;        // padded width
+  int n_hidden;
+  int dims[17];      // layer widths: dims[0] = 3, dims[k] = hidden[k-1], dims[H+1] = 1
+  int64_t woff[17];                  // offset of W_k in W
+  int64_t boff[17];                  // offset of b_k in B
+  float4* w0v;
+  void* apk;
+  float* biasc;
+  float* wlc;
+  float* wlv;
+  float* blp;
+};
+
+__global__ void k_pack_small(PackArgs a) {
+  const ShapeInfo s = shape_info(a.shape);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int H = a.n_hidden;
+  if (i < a.width) {
+    // layer 0 in virtual order
+    const int u = shape_perm(s, i);
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (u < a.dims[1]) {
+      const float* row = a.W + a.woff[0] + (int64_t)u * 3;
+      w = make_float4(row[0], row[1], row[2], a.B[a.boff[0] + u]);
+    }
+    a.w0v[i] = w;
+    // output layer, virtual order
+    a.wlv[i] = (u < a.dims[H]) ? a.W[a.woff[H] + u] : 0.f;
+    // output layer, C order: i = (rb * GROUPS + g) * CREGS + q
+    const int q = i % s.cregs, g = (i / s.cregs) % s.groups, rb = i / (s.cregs * s.groups);
+    const int r = rb * s.rows + shape_rowmap(s, g, q);
+    a.wlc[i] = (r < a.dims[H]) ? a.W[a.woff[H] + r] : 0.f;
+    // hidden biases, C order
+    for (int j = 1; j < H; ++j) a.biasc[(int64_t)(j - 1) * a.width + i] = (r < a.dims[j + 1]) ? a.B[a.boff[j] + r] : 0.f;
+  }
+  if (i == 0) *a.blp = a.B[a.boff[H]];
+}
+
+// Hidden layer j (1..H-1): element ((rb * KBS + kb) * 64 + lane) * EPL + e.
+__global__ void k_pack_hidden(PackArgs a, int j) {
+  const ShapeInfo s = shape_info(a.shape);
+  const int64_t per_layer = (int64_t)a.width * a.width;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per_layer) return;
+  const int e = (int)(i % s.epl);
+  const int lane = (int)((i / s.epl) % 64);
+  const int64_t blk = i / (s.epl * 64);
+  const int kbs = a.width / s.kb;
+  const int kb = (int)(blk % kbs), rb = (int)(blk / kbs);
+  const int r = rb * s.rows + lane % s.rows;
+  const int v = kb * s.kb + (lane / s.rows) * s.epl + e;
+  const int u = shape_perm(s, v);
+  const int out_j = a.dims[j + 1], in_j = a.dims[j];
+  const float w = (r < out_j && u < in_j) ? a.W[a.woff[j] + (int64_t)r * in_j + u] : 0.f;
+  const int64_t o = (int64_t)(j - 1) * per_layer + i;
+  switch (a.shape) {
+    case SH_32x32x16_BF16: case SH_16x16x32_BF16: reinterpret_cast<__bf16*>(a.apk)[o] = (__bf16)w; break;
+    case SH_32x32x16_F16: case SH_16x16x32_F16: reinterpret_cast<_Float16*>(a.apk)[o] = (_Float16)w; break;
+    default: reinterpret_cast<float*>(a.apk)[o] = w; break;
+  }
+}
+
+static int pack_level(const mp_stack* st, const mp_stack::Level& L, hipStream_t stream) {
+  PackArgs a{};
+  a.W = L.weights;
+  a.B = L.biases;
+  a.shape = st->shape;
+  a.width = L.width;
+  a.n_hidden = L.n_hidden;
+  a.dims[0] = 3;
+  for (int k = 0; k < L.n_hidden; ++k) a.dims[k + 1] = L.hidden[k];
+  a.dims[L.n_hidden + 1] = 1;
+  int64_t wo = 0, bo = 0;
+  for (int k = 0; k <= L.n_hidden; ++k) {
+    a.woff[k] = wo;
+    a.boff[k] = bo;
+    wo += (int64_t)a.dims[k + 1] * a.dims[k];
+    bo += a.dims[k + 1];
+  }
+  a.w0v = const_cast<float4*>(L.dev.w0v);
+  a.apk = const_cast<void*>(L.dev.apk);
+  a.biasc = const_cast<float*>(L.dev.biasc);
+  a.wlc = const_cast<float*>(L.dev.wlc);
+  a.wlv = const_cast<float*>(L.dev.wlv);
+  a.blp = const_cast<float*>(L.dev.blp);
+  hipLaunchKernelGGL(k_pack_small, dim3((L.width + 255) / 256), dim3(256), 0, stream, a);
+  MP_LAUNCHED("k_pack_small");
+  const int64_t per_layer = (int64_t)L.width * L.width;
+  for (int j = 1; j < L.n_hidden; ++j) {
+    hipLaunchKernelGGL(k_pack_hidden, dim3((unsigned)((per_layer + 255) / 256)), dim3(256), 0,
+                       stream, a, j);
+    MP_LAUNCHED("k_pack_hidden");
+  }
+  return MP_OK;
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+extern "C" {
+
+int mp_abi_version(void) { return MP_ABI_VERSION; }
+
+const char* mp_last_error(void) { return mp::get_error(); }
+
+int mp_stack_levels(const mp_stack* stack) { return stack ? stack->n_levels : 0; }
+
+void mp_stack_destroy(mp_stack* st) {
+  if (!st) return;
+  for (int l = 0; l < st->n_levels; ++l)
+    if (st->lv[l].buf) (void)hipFree(st->lv[l].buf);
+  if (st->scratch) (void)hipFree(st->scratch);
+  delete st;
+}
+
+int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, void* stream,
+                    mp_stack** out) {
+  MP_REQUIRE(out != nullptr, MP_EINVAL, "mp_stack_create: out is NULL");
+  *out = nullptr;
+  MP_REQUIRE(levels != nullptr && n_levels >= 1 && n_levels <= MP_MAX_LEVELS, MP_EINVAL,
+             "mp_stack_create: n_levels must be in [1, %d], got %d", MP_MAX_LEVELS, n_levels);
+  MP_REQUIRE(precision == MP_FP32 || precision == MP_BF16 || precision == MP_FP16, MP_EINVAL,
+             "mp_stack_create: unknown precision %d", precision);
+  mp_stack* st = new mp_stack();
+  st->n_levels = n_levels;
+  st->precision = precision;
+  st->shape = precision == MP_BF16 ? SH_32x32x16_BF16
+              : precision == MP_FP16 ? SH_32x32x16_F16
+                                     : SH_16x16x4_F32;
+  const ShapeInfo si = shape_info(st->shape);
+  int maxw = 0;
+  for (int l = 0; l < n_levels; ++l) {
+    const mp_level_desc& d = levels[l];
+    mp_stack::Level& L = st->lv[l];
+    if (d.n_in != 3 || d.n_out != 1 || d.n_hidden < 1 || d.n_hidden > 15 || !d.hidden ||
+        !d.weights || !d.biases) {
+      mp::set_error("mp_stack_create: level %d: need n_in=3, n_out=1, 1<=n_hidden<=15 and "
+                    "non-NULL hidden/weights/biases (got n_in=%d n_out=%d n_hidden=%d)",
+                    l, d.n_in, d.n_out, d.n_hidden);
+      mp_stack_destroy(st);
+      return MP_EINVAL;
+    }
+    int w = 0;
+    for (int k = 0; k < d.n_hidden; ++k) {
+      if (d.hidden[k] < 1) {
+        mp::set_error("mp_stack_create: level %d: hidden width %d is < 1", l, d.hidden[k]);
+        mp_stack_destroy(st);
+        return MP_EINVAL;
+      }
+      w = d.hidden[k] > w ? d.hidden[k] : w;
+      L.hidden.push_back(d.hidden[k]);
+    }
+    L.width = padded_width(w);
+    if (L.width < 0) {
+      mp::set_error("mp_stack_create: level %d: width %d exceeds the supported maximum 512", l, w);
+      mp_stack_destroy(st);
+      return MP_EUNSUPPORTED;
+    }
+    maxw = L.width > maxw ? L.width : maxw;
+    L.n_hidden = d.n_hidden;
+    L.weights = d.weights;
+    L.biases = d.biases;
+    const size_t W = (size_t)L.width;
+    const size_t n_gemm = (size_t)(L.n_hidden - 1);
+    const size_t sz_w0v = W * sizeof(float4);
+    const size_t sz_apk = n_gemm * W * W * si.elem_bytes;
+    const size_t sz_bias = n_gemm * W * sizeof(float);
+    const size_t sz_wl = W * sizeof(float);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = al(sz_w0v) + al(sz_apk) + al(sz_bias) + 2 * al(sz_wl) + 256;
+    hipError_t e = hipMalloc(&L.buf, total);
+    if (e != hipSuccess) {
+      mp::set_error("mp_stack_create: hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
+      mp_stack_destroy(st);
+      return MP_ENOMEM;
+    }
+    L.buf_bytes = total;
+    char* p = reinterpret_cast<char*>(L.buf);
+    L.dev.width = L.width;
+    L.dev.n_hidden = L.n_hidden;
+    L.dev.w0v = reinterpret_cast<const float4*>(p); p += al(sz_w0v);
+    L.dev.apk = p; p += al(sz_apk);
+    L.dev.biasc = reinterpret_cast<const float*>(p); p += al(sz_bias);
+    L.dev.wlc = reinterpret_cast<const float*>(p); p += al(sz_wl);
+    L.dev.wlv = reinterpret_cast<const float*>(p); p += al(sz_wl);
+    L.dev.blp = reinterpret_cast<const float*>(p);
+  }
+  st->nc = default_nc(st->shape, maxw);
+  int rc = mp_stack_update(st, stream);
+  if (rc != MP_OK) {
+    mp_stack_destroy(st);
+    return rc;
+  }
+  *out = st;
+  return MP_OK;
+}
+
+int mp_stack_update(mp_stack* st, void* stream) {
+  MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_stack_update: stack is NULL");
+  for (int l = 0; l < st->n_levels; ++l) {
+    int rc = pack_level(st, st->lv[l], as_stream(stream));
+    if (rc != MP_OK) return rc;
+  }
+  return MP_OK;
+}
+
+int mp_eval(const mp_stack* st, const float* xyz, int64_t n, int lvl0, int lvl1, float* sdf,
+            float* grad, void* stream) {
+  MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_eval: stack is NULL");
+  MP_REQUIRE(n >= 0, MP_EINVAL, "mp_eval: n must be >= 0, got %lld", (long long)n);
+  MP_REQUIRE(0 <= lvl0 && lvl0 < lvl1 && lvl1 <= st->n_levels, MP_EINVAL,
+             "mp_eval: need 0 <= lvl0 < lvl1 <= %d, got [%d, %d)", st->n_levels, lvl0, lvl1);
+  if (n == 0) return MP_OK;
+  MP_REQUIRE(xyz != nullptr && sdf != nullptr, MP_EINVAL, "mp_eval: xyz and sdf must be non-NULL");
+  for (int l = lvl0; l < lvl1; ++l) {
+    EvalTarget t{};
+    t.src = SRC_POINTS;
+    t.xyz = xyz;
+    t.n = n;
+    t.out = sdf;
+    t.accumulate = l > lvl0;
+    int rc = launch_level(st, l, t, as_stream(stream));
+    if (rc != MP_OK) return rc;
+  }
+  if (grad) {
+    int rc = launch_grad(st, lvl0, lvl1, xyz, n, grad, as_stream(stream));
+    if (rc != MP_OK) return rc;
+  }
+  return MP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+"""ctypes binding of libmplicits_hip.so (the C ABI declared in include/mplicits.h).
+
+The library is loaded from this package directory.  There is no fallback: if it
+is missing or cannot be loaded, every GPU entry point raises.
+"""
+import ctypes
+import os.path as osp
+import threading
+
+import torch
+
+LIB_PATH = osp.join(osp.dirname(osp.abspath(__file__)), "libmplicits_hip.so")
+
+MP_OK = 0
+MP_EINVAL = -1
+MP_ENOMEM = -2
+MP_EHIP = -3
+MP_EUNSUPPORTED = -4
+
+MP_FP32 = 0
+MP_BF16 = 1
+MP_FP16 = 2
+PRECISIONS = {"fp32": MP_FP32, "bf16": MP_BF16, "fp16": MP_FP16}
+
+MP_GRID_REFERENCE = 0
+MP_GRID_LATTICE = 1
+GRID_MODES = {"reference": MP_GRID_REFERENCE, "lattice": MP_GRID_LATTICE}
+
+MP_MAX_LEVELS = 4
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+
+class mp_level_desc(ctypes.Structure):
+    _fields_ = [
+        ("n_in", ctypes.c_int),
+        ("n_out", ctypes.c_int),
+        ("n_hidden", ctypes.c_int),
+        ("hidden", ctypes.POINTER(ctypes.c_int)),
+        ("weights", ctypes.c_void_p),
+        ("biases", ctypes.c_void_p),
+    ]
+
+
+# Exported symbols: name -> (restype, argtypes).  The CPU tests check that the
+# library exports every one of them.
+SIGNATURES = {
+    "mp_abi_version": (ctypes.c_int, []),
+    "mp_last_error": (ctypes.c_char_p, []),
+    "mp_stack_create": (ctypes.c_int, [ctypes.POINTER(mp_level_desc), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "mp_stack_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_stack_destroy": (None, [ctypes.c_void_p]),
+    "mp_stack_levels": (ctypes.c_int, [ctypes.c_void_p]),
+    "mp_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_extract_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                         c_float_p, ctypes.c_int, ctypes.c_void_p, c_int64_p,
+                                         ctypes.c_void_p]),
+    "mp_grid_coords": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class MPError(RuntimeError):
+    """A nonzero status from the HIP library."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+def load():
+    """Load the library (once).  Raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not osp.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"libmplicits_hip.so not found at {LIB_PATH}; build it with "
+                    "`python m-plicits_amd/build_lib.py` (or __graft_entry__.build())")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(code):
+    """Map a status code to the exception types of the reference."""
+    if code == MP_OK:
+        return
+    msg = load().mp_last_error().decode(errors="replace")
+    if code == MP_EINVAL:
+        raise ValueError(msg)
+    if code == MP_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    if code == MP_ENOMEM:
+        raise MemoryError(msg)
+    raise MPError(code, msg)
+
+
+def stream_handle(device=None):
+    """hipStream_t of torch's current stream, as a void pointer."""
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def require_gpu_tensor(t, name):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise RuntimeError(
+            f"{name} must be a tensor on the GPU: the mplicits hot path runs only through "
+            "the HIP library (no CPU fallback)")

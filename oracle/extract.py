@@ -1,0 +1,78 @@
+"""Band-culled sampling loop of create_mesh_multistage (oracle; test infrastructure only).
+
+restates i3d/meshing.py:203-238 per chunk of `max_batch` consecutive linear
+indices (reconstruct.py:128 passes 32**3):
+    s = f_0(x)
+    for i in 1..L-1:
+        if deltas:
+            m = |s| < delta_{i-1};  if not m.any(): break;  s[m] += f_i(x[m])
+        else:
+            s += f_i(x)
+The comparisons use float32 deltas, as torch compares a float32 tensor with a
+Python float.
+"""
+import numpy as np
+import torch
+
+from . import grid as ogrid
+from . import siren as osiren
+
+
+def sample_chunk(level_layers, xyz, deltas, dtype=np.float64):
+    """Restated loop body for one chunk; level_layers are oracle layer lists."""
+    s = osiren.forward(level_layers[0], xyz).astype(dtype)
+    for i in range(1, len(level_layers)):
+        if deltas:
+            m = np.abs(s) < np.float32(deltas[i - 1])
+            if not m.any():
+                break
+            s[m] = s[m] + osiren.forward(level_layers[i], xyz[m]).astype(dtype)
+        else:
+            s = s + osiren.forward(level_layers[i], xyz).astype(dtype)
+    return s
+
+
+def sample_chunks(level_layers, N, deltas, max_batch, chunk_ids, mode="reference"):
+    """{chunk id: (indices, values)} for the requested chunks (FP64 evaluation)."""
+    out = {}
+    total = N ** 3
+    for c in chunk_ids:
+        idx = np.arange(c * max_batch, min((c + 1) * max_batch, total), dtype=np.int64)
+        xyz = ogrid.coords(N, idx, mode).astype(np.float64)
+        out[c] = (idx, sample_chunk(level_layers, xyz, deltas))
+    return out
+
+
+def torch_sampling_loop(level_layers, N, deltas, max_batch, chunk_ids=None, mode="reference",
+                        threads=None):
+    """The reference loop in torch float32 on the CPU (the CPU baseline).
+
+    level_layers: per level [(W, b)] float32 torch tensors (baked).  Evaluates
+    the listed chunks (all when None) and returns (values by chunk, per-level
+    evaluation counts)."""
+    if threads:
+        torch.set_num_threads(threads)
+    total = N ** 3
+    n_chunks = (total + max_batch - 1) // max_batch
+    chunk_ids = range(n_chunks) if chunk_ids is None else chunk_ids
+    counts = [0] * len(level_layers)
+    out = {}
+    with torch.no_grad():
+        for c in chunk_ids:
+            idx = np.arange(c * max_batch, min((c + 1) * max_batch, total), dtype=np.int64)
+            x = torch.from_numpy(ogrid.coords(N, idx, mode))
+            s = osiren.torch_forward(level_layers[0], x).squeeze(-1)
+            counts[0] += x.shape[0]
+            for i in range(1, len(level_layers)):
+                if deltas:
+                    m = torch.abs(s) < deltas[i - 1]
+                    if not m.any():
+                        break
+                    xm = x[m]
+                    counts[i] += xm.shape[0]
+                    s[m] += osiren.torch_forward(level_layers[i], xm).squeeze(-1)
+                else:
+                    counts[i] += x.shape[0]
+                    s += osiren.torch_forward(level_layers[i], x).squeeze(-1)
+            out[c] = (idx, s.numpy())
+    return out, counts

@@ -1,0 +1,88 @@
+"""Fused level kernel (mp_eval) against the FP64 oracle (restated i3d SIREN forward)."""
+import numpy as np
+import pytest
+import torch
+
+from mplicits.stack import LevelStack, grid_coords
+from mplicits.synthetic import seeded_siren
+from oracle import grid as ogrid
+from oracle import siren as osiren
+
+pytestmark = pytest.mark.gpu
+
+# (hidden widths, omega, last-layer scale): padded widths 64..512, ragged widths,
+# one sine layer (no hidden GEMM) and a deeper stack (ping-pong path).
+CASES = [
+    ([64, 64], 30.0, 1.0),
+    ([128, 128, 128], 30.0, 1.0),
+    ([256, 256, 256], 30.0, 1.0),
+    ([512, 512, 512], 90.0, 0.1),
+    ([100, 200, 50], 30.0, 1.0),
+    ([256], 30.0, 1.0),
+    ([256, 256, 256, 256, 256], 30.0, 1.0),
+]
+# Max |f_gpu - f_fp64| relative to max |f_fp64| over the sample.
+REL_TOL = {"fp32": 2e-5, "fp16": 2e-3, "bf16": 1e-2}
+
+
+def _points(n, seed):
+    g = torch.Generator().manual_seed(seed)
+    return torch.rand(n, 3, generator=g, dtype=torch.float64) * 2 - 1
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_forward_matches_fp64_oracle(case, precision):
+    hidden, omega, last = case
+    m = seeded_siren(hidden, omega, seed=1234 + len(hidden) + hidden[0], last_scale=last).cuda()
+    x64 = _points(10007, 7)  # ragged: not a multiple of any tile
+    ref = osiren.forward(osiren.layers_from_model(m), x64.numpy())
+    st = LevelStack([m], precision)
+    f = st.eval(x64.float().cuda()).double().cpu().numpy()
+    err = np.abs(f - ref).max()
+    scale = np.abs(ref).max()
+    print(f"{precision} {hidden}: max err {err:.3e}, max |f| {scale:.3e}")
+    assert err <= REL_TOL[precision] * scale
+
+
+def test_fp32_absolute_error_on_unit_outputs():
+    # A baked w=30 SIREN whose outputs are O(1): FP32 path within 1e-5 absolute.
+    m = seeded_siren([256, 256, 256], 30.0, seed=99, last_scale=30.0).cuda()
+    x64 = _points(20000, 3)
+    ref = osiren.forward(osiren.layers_from_model(m), x64.numpy())
+    f = LevelStack([m], "fp32").eval(x64.float().cuda()).double().cpu().numpy()
+    assert np.abs(ref).max() > 0.3
+    assert np.abs(f - ref).max() <= 1e-5
+
+
+def test_level_sum_and_empty_input():
+    levels = [seeded_siren([128, 128], 30.0, seed=s).cuda() for s in (1, 2, 3)]
+    st = LevelStack(levels, "fp32")
+    x64 = _points(513, 11)
+    ref = sum(osiren.forward(osiren.layers_from_model(m), x64.numpy()) for m in levels[1:])
+    f = st.eval(x64.float().cuda(), lvl0=1, lvl1=3).double().cpu().numpy()
+    assert np.abs(f - ref).max() <= 1e-5
+    empty = st.eval(torch.zeros(0, 3, device="cuda"))
+    assert empty.shape == (0,)
+    with pytest.raises(ValueError):
+        st.eval(x64.float().cuda(), lvl0=2, lvl1=1)
+
+
+def test_weights_update_repacks():
+    m = seeded_siren([64, 64], 30.0, seed=5).cuda()
+    st = LevelStack([m], "fp32")
+    x = _points(300, 2).float().cuda()
+    f0 = st.eval(x).clone()
+    with torch.no_grad():
+        m.net[-1][0].bias.add_(0.25)
+    f1 = st.eval(x)
+    assert torch.allclose(f1, f0 + 0.25, atol=1e-6)
+
+
+@pytest.mark.parametrize("mode", ["reference", "lattice"])
+@pytest.mark.parametrize("N", [4, 33, 512])
+def test_grid_coords_bit_exact(N, mode):
+    total = N ** 3
+    first = max(0, total - 70000) if N == 512 else 0
+    xyz = grid_coords(N, mode, first=first, n=total - first).cpu().numpy()
+    np.testing.assert_array_equal(xyz, ogrid.coords(N, np.arange(first, total), mode))
