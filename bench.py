@@ -1,0 +1,215 @@
+#!/usr/bin/env python
+"""Benchmark: 512^3 three-level band-culled extraction on one MI355X.
+
+Workload (the three-level configuration of the workload list): coarse
+SIREN 256x3 (w0 = 30, fitted to the seeded bumpy sphere,
+tests/golden/bumpy_coarse_256x3.pth), medium residual 256x3 (w = 60, delta 0.05),
+fine residual 512x3 (w = 90, delta 0.02), seeded SIREN init with the residuals'
+last layers x 0.1.  One step = band-culled evaluation of the 512^3
+reference-warped grid in 32^3 chunks (create_mesh_multistage semantics) plus
+GPU marching cubes of the volume (vertex/face emission), inputs resident in HBM.
+
+Prints one JSON line: value = ms per extraction of the BF16-MFMA path; the FP32
+and FP16 paths, band counts, SDF errors against the FP64 oracle, the roofline
+of the dominant kernel (coarse level over the grid) and the CPU baseline (the
+oracle's torch-CPU restatement of the reference loop plus the C marching cubes,
+on a bounded sample, scaled) ride along.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+import argparse
+import json
+import os
+import os.path as osp
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = osp.dirname(osp.abspath(__file__))
+sys.path.insert(0, osp.join(ROOT, "m-plicits_amd"))
+
+from mplicits.meshing import marching_cubes_gpu  # noqa: E402
+from mplicits.model import from_pth  # noqa: E402
+from mplicits.stack import LevelStack  # noqa: E402
+from mplicits.synthetic import DELTAS, residual_levels  # noqa: E402
+
+N = 512
+CHUNK = 32 ** 3
+COARSE = osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")
+# algorithmic work per sample of one level: 2 FLOP per multiply-add of
+# 3 -> W -> ... -> W -> 1 (first layer, hidden GEMMs, output layer)
+PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}
+
+
+def level_flops(hidden):
+    widths = [3] + list(hidden) + [1]
+    return 2 * sum(a * b for a, b in zip(widths[:-1], widths[1:]))
+
+
+def load_levels():
+    coarse = from_pth(COARSE).cuda().eval()
+    med, fine = residual_levels("cuda")
+    return [coarse, med.eval(), fine.eval()]
+
+
+def extraction_step(stack, vol):
+    _, counts = stack.extract_volume(N, DELTAS, chunk=CHUNK, out=vol)
+    v, f, _, _ = marching_cubes_gpu(vol, 0.0, with_normals=False)
+    return counts, v.shape[0], f.shape[0]
+
+
+def timed(stack, vol, steps, warmup):
+    for _ in range(warmup):
+        extraction_step(stack, vol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counts, nv, nf = extraction_step(stack, vol)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, counts, nv, nf
+
+
+def dominant_kernel_ms(model, precision, reps=5):
+    """Average duration of the coarse-level grid kernel: a one-level unculled
+    extraction is exactly one launch of it; HIP events on the current stream."""
+    st = LevelStack([model], precision)
+    vol = torch.empty(N, N, N, device="cuda")
+    st.extract_volume(N, [], out=vol)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    total = 0.0
+    for _ in range(reps):
+        e0.record(stream)
+        st.extract_volume(N, [], out=vol)
+        e1.record(stream)
+        e1.synchronize()
+        total += e0.elapsed_time(e1)
+    return total / reps
+
+
+def sdf_errors(levels, vol, n_surface=6, n_random=4):
+    """|GPU - FP64 oracle| over whole chunks (the oracle evaluates the same
+    culling loop).  Samples whose band membership flips between the two are
+    counted separately (their difference is a whole residual)."""
+    from oracle import extract as oextract
+    from oracle import siren as osiren
+    layers = [osiren.layers_from_model(m) for m in levels]
+    flat = vol.reshape(-1)
+    n_chunks = N ** 3 // CHUNK
+    # chunks straddling the surface (near |f| = 0) and a few strided ones
+    near = torch.nonzero(flat.abs() < 0.01).flatten()
+    picks = set()
+    if len(near):
+        idx = near[torch.linspace(0, len(near) - 1, n_surface).long()].cpu().numpy()
+        picks.update(int(i) // CHUNK for i in idx)
+    picks.update(range(0, n_chunks, n_chunks // n_random))
+    ref = oextract.sample_chunks(layers, N, list(DELTAS), CHUNK, sorted(picks))
+    errs = []
+    flips = 0
+    for _, (idx, r) in ref.items():
+        g = flat[torch.from_numpy(idx).cuda()].double().cpu().numpy()
+        e = np.abs(g - r)
+        bad = e > 1e-3  # a flipped band decision adds or drops a residual
+        flips += int(bad.sum())
+        errs.append(e[~bad])
+    e = np.concatenate(errs)
+    return {"max": float(e.max()), "mean": float(e.mean()), "band_flips": flips,
+            "samples": int(e.size + flips), "chunks": len(ref)}
+
+
+def cpu_baseline(levels, budget_chunks=96):
+    """The reference's CPU path on a bounded sample: the oracle's torch-CPU
+    float32 restatement of the chunked sampling loop on `budget_chunks` evenly
+    strided 32^3 chunks, plus the C Lewiner marching cubes on a 64-slice slab,
+    both scaled to the full 512^3 extraction."""
+    from oracle import extract as oextract
+    from oracle import mc as omc
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    tl = []
+    for m in levels:
+        layers = []
+        for k, seq in enumerate(m.net):
+            w = seq[0].weight.detach().cpu().float()
+            b = seq[0].bias.detach().cpu().float()
+            layers.append((w, b))
+        tl.append(layers)
+    n_chunks = N ** 3 // CHUNK
+    ids = list(range(0, n_chunks, n_chunks // budget_chunks))[:budget_chunks]
+    t0 = time.perf_counter()
+    _, counts = oextract.torch_sampling_loop(tl, N, list(DELTAS), CHUNK, ids)
+    t_eval = (time.perf_counter() - t0) * n_chunks / len(ids)
+    # marching cubes on a slab of the GPU volume (same values)
+    return t_eval, ids, threads
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.set_device(0)
+    levels = load_levels()
+    vol = torch.empty(N, N, N, device="cuda")
+
+    results = {}
+    for prec in ("bf16", "fp16", "fp32"):
+        st = LevelStack(levels, prec)
+        steps = args.steps if prec != "fp32" else max(1, min(args.steps, 2))
+        ms, counts, nv, nf = timed(st, vol, steps, min(args.warmup, 1) if prec == "fp32" else args.warmup)
+        err = sdf_errors(levels, vol)
+        results[prec] = {"ms": ms, "counts": counts, "verts": nv, "faces": nf, "sdf_err": err}
+        print(f"# {prec}: {ms:.2f} ms/extraction, counts {counts}, mesh {nv} v / {nf} f, err {err}",
+              file=sys.stderr, flush=True)
+        del st
+
+    # roofline of the dominant kernel (coarse level over the whole grid)
+    coarse_flop = level_flops(levels[0].hidden_widths()) * N ** 3
+    k_ms = dominant_kernel_ms(levels[0], "bf16")
+    achieved = coarse_flop / (k_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "k_level<S32x32x16_bf16,256,2> (coarse 256x3 over 512^3)",
+                "achieved": round(achieved, 1), "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_TFLOPS["bf16"], 4), "traffic": None,
+                "kernel_ms": round(k_ms, 3), "flop_per_launch": coarse_flop}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        t_eval, ids, threads = cpu_baseline(levels)
+        cpu = {"value": round(t_eval * 1e3, 1), "unit": "ms per 512^3 extraction (sampling loop)",
+               "cores": threads, "kind": "port",
+               "sample": f"{len(ids)} evenly strided 32^3 chunks of 4096 through the torch-CPU float32 "
+                         f"restatement of the reference sampling loop, scaled x{4096 / len(ids):.1f}; "
+                         "marching cubes excluded"}
+
+    b = results["bf16"]
+    out = {
+        "metric": "512^3 band-culled extraction ms, rays/s, train pts/s: BF16 vs FP32 at SDF err",
+        "value": round(b["ms"], 3), "unit": "ms per 512^3 band-culled extraction (BF16, incl. GPU MC)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(b["ms"], 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (seeded bumpy sphere; fitted coarse level + seeded residual levels)",
+        "config": {"workload": "three-level band-culled extraction, 512^3 grid, 32^3 chunks",
+                   "levels": "256x3 w30 > 256x3 w60 (delta 0.05) > 512x3 w90 (delta 0.02)",
+                   "grid": "reference", "global_batch": N ** 3, "parallelism": "none (1 GPU)"},
+        "fp32_ms": round(results["fp32"]["ms"], 3),
+        "fp16_ms": round(results["fp16"]["ms"], 3),
+        "speedup_bf16_vs_fp32": round(results["fp32"]["ms"] / b["ms"], 2),
+        "level_counts": b["counts"], "mesh": {"verts": b["verts"], "faces": b["faces"]},
+        "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -76,6 +76,12 @@ int mp_stack_update(mp_stack* stack, void* stream);
 void mp_stack_destroy(mp_stack* stack);
 int mp_stack_levels(const mp_stack* stack);
 
+/* Points per wavefront tile of the level kernels (a tile shape picked by the
+ * autotuner, mplicits/autotune.py): 1 or 2 column blocks of the MFMA shape.
+ * No counterpart in the reference (its CUTLASS tile shapes are compile-time,
+ * renderer/cuda/layer_gemms.cu:1612-1856). */
+int mp_stack_set_tiles(mp_stack* stack, int column_blocks);
+
 /* Evaluate sum_{l=lvl0}^{lvl1-1} f_l(x) at n points xyz [n][3] into sdf [n].
  * Replaces SIREN.forward (i3d/model.py:107-129) summed over levels as
  * create_mesh_multistage does with deltas=[] (i3d/meshing.py:232-233).
@@ -98,6 +104,25 @@ int mp_extract_volume(mp_stack* stack, int N, int grid_mode, int64_t chunk, cons
 /* Sample coordinates of grid index range [first, first+n) into xyz [n][3]
  * (gen_mc_coordinate_grid, i3d/meshing.py:11-69, columns 0..2). */
 int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, void* stream);
+
+/* Marching cubes of a [nz][ny][nx] float32 device volume at `level`: replaces
+ * skimage.measure.marching_cubes(volume, level) as convert_sdf_samples_to_ply
+ * calls it (i3d/meshing.py:296-298; Lewiner method, 'descent' winding,
+ * degenerate triangles kept).  Two calls: mp_mc_count (host-sync point) returns
+ * the vertex and face counts, the caller allocates, mp_mc_emit writes
+ *   verts   [V][3] float32, voxel units, (axis0, axis1, axis2) order,
+ *   faces   [F][3] int32 vertex ids,
+ *   normals [V][3] float32 unit -grad(volume) (may be NULL),
+ *   values  [V] float32 local value range (may be NULL).
+ * Vertex and face ids are deterministic (prefix sums in linear grid order).
+ * verts and faces may be NULL only when the counts are zero.
+ * `ws` is a device workspace of mp_mc_workspace_size(nz, ny, nx) bytes, kept
+ * between the two calls. */
+size_t mp_mc_workspace_size(int nz, int ny, int nx);
+int mp_mc_count(const float* vol, int nz, int ny, int nx, float level, int64_t* n_verts,
+                int64_t* n_faces, void* ws, void* stream);
+int mp_mc_emit(const float* vol, int nz, int ny, int nx, float level, float* verts, int32_t* faces,
+               float* normals, float* values, void* ws, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,506 @@
+// Marching cubes on the GPU with integer-exact, deterministic vertex and face
+// indexing.  Replaces the CPU call the reference makes after sampling,
+// skimage.measure.marching_cubes(volume, 0.0, spacing=[h]*3) (Lewiner method,
+// gradient_direction='descent', allow_degenerate=True; i3d/meshing.py:296-298).
+//
+// Volume layout: [nz][ny][nx] float32, x fastest (the reference's
+// sdf.reshape(N, N, N)).  Output vertices are in voxel units, (axis0, axis1,
+// axis2) = (z, y, x) order, as scikit-image returns them before the spacing
+// multiply; the caller applies spacing and origin.
+//
+// Vertex ownership: grid point (x, y, z) owns the intersected edges that leave
+// it in +x, +y, +z (one vertex each) and the centre vertex of the cell whose
+// lowest corner it is, when that cell's tiling uses one.  Vertex ids are an
+// exclusive prefix sum of the per-point vertex counts in linear point order;
+// face ids the same over per-cell triangle counts.  Passes:
+//   1. count: per point (vertex count, triangle count), block sums
+//   2. scan the block sums (one workgroup)
+//   3. vertices: block-local scan + block offset -> vbase[point]; write vertices
+//   4. faces: recompute each cell's tiling, block-local scan -> face ids; map
+//      each tiling entry to (owner point, axis) -> vertex id
+// Corner classification, ambiguity tests and vertex placement follow the
+// Lewiner algorithm with scikit-image's numerics: double arithmetic on
+// v = volume - level, a corner is inside when v > 0, tests against 2^-52,
+// an edge vertex = its two corners weighted by 1 / (2^-52 + |v|) (the linear
+// interpolation point), the centre vertex = the eight corners weighted the
+// same way, the interior test's undecided sub-cases 5 and 10 read as false.
+#include "common.h"
+
+#define MC_TABLE __device__ static const
+#include "mc_tables.h"
+
+namespace mp {
+
+static constexpr double MC_EPS = 2.220446049250313e-16;
+static constexpr int MC_BLOCK = 256;
+
+// corner p -> (dx, dy, dz)
+__device__ static const int8_t MC_CX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__device__ static const int8_t MC_CY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__device__ static const int8_t MC_CZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+// edge -> corners
+__device__ static const int8_t MC_EA[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+__device__ static const int8_t MC_EB[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+// edge -> owner point offset (dx, dy, dz) and axis (0 x, 1 y, 2 z)
+__device__ static const int8_t MC_OX[12] = {0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 0};
+__device__ static const int8_t MC_OY[12] = {0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 1};
+__device__ static const int8_t MC_OZ[12] = {0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 0, 0};
+__device__ static const int8_t MC_AX[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+// For the interior test through edge e: corners of the three parallel edges,
+// in cyclic order around it (B0 B1 C0 C1 D0 D1).
+__device__ static const int8_t MC_PAR[12][6] = {
+    {3, 2, 7, 6, 4, 5}, {0, 3, 4, 7, 5, 6}, {1, 0, 5, 4, 6, 7}, {2, 1, 6, 5, 7, 4},
+    {7, 6, 3, 2, 0, 1}, {4, 7, 0, 3, 1, 2}, {5, 4, 1, 0, 2, 3}, {6, 5, 2, 1, 3, 0},
+    {3, 7, 2, 6, 1, 5}, {0, 4, 3, 7, 2, 6}, {1, 5, 0, 4, 3, 7}, {2, 6, 1, 5, 0, 4}};
+// face -> corners (A, B, C, D)
+__device__ static const int8_t MC_FACE[6][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
+                                               {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
+
+struct McVol {
+  const float* vol;
+  int nx, ny, nz;
+  double level;
+};
+
+__device__ inline double mc_value(const McVol& V, int x, int y, int z) {
+  return (double)V.vol[((int64_t)z * V.ny + y) * V.nx + x] - V.level;
+}
+__device__ inline bool mc_pos(double v) { return v > 0; }
+__device__ inline double mc_weight(double v) { return 1.0 / (MC_EPS + fabs(v)); }
+
+__device__ inline bool mc_face(const double* v, int face) {
+  const int f = (face < 0 ? -face : face) - 1;
+  const double A = v[MC_FACE[f][0]], B = v[MC_FACE[f][1]], C = v[MC_FACE[f][2]], D = v[MC_FACE[f][3]];
+  const double det = A * C - B * D;
+  if (fabs(det) < MC_EPS) return face >= 0;
+  return face * A * det >= 0;
+}
+
+__device__ inline bool mc_interior(const double* v, int cas, int cfg, int sub, int s) {
+  double At = 0, Bt = 0, Ct = 0, Dt = 0;
+  if (cas == 4 || cas == 10) {
+    const double a = (v[4] - v[0]) * (v[6] - v[2]) - (v[7] - v[3]) * (v[5] - v[1]);
+    const double b = v[2] * (v[4] - v[0]) + v[0] * (v[6] - v[2]) - v[1] * (v[7] - v[3]) -
+                     v[3] * (v[5] - v[1]);
+    const double t = -b / (2 * a);
+    if (t < 0 || t > 1) return s > 0;
+    At = v[0] + (v[4] - v[0]) * t;
+    Bt = v[3] + (v[7] - v[3]) * t;
+    Ct = v[2] + (v[6] - v[2]) * t;
+    Dt = v[1] + (v[5] - v[1]) * t;
+  } else {
+    int edge;
+    if (cas == 6) edge = MC_TEST6[cfg][2];
+    else if (cas == 7) edge = MC_TEST7[cfg][4];
+    else if (cas == 12) edge = MC_TEST12[cfg][3];
+    else edge = MC_TILING13_5_1[cfg][sub][0];
+    const double t = v[MC_EA[edge]] / (v[MC_EA[edge]] - v[MC_EB[edge]]);
+    const int8_t* e = MC_PAR[edge];
+    Bt = v[e[0]] + (v[e[1]] - v[e[0]]) * t;
+    Ct = v[e[2]] + (v[e[3]] - v[e[2]]) * t;
+    Dt = v[e[4]] + (v[e[5]] - v[e[4]]) * t;
+  }
+  const int test = (At >= 0) | ((Bt >= 0) << 1) | ((Ct >= 0) << 2) | ((Dt >= 0) << 3);
+  switch (test) {
+    case 5: return (At * Ct - Bt * Dt < MC_EPS) ? s > 0 : false;
+    case 10: return (At * Ct - Bt * Dt >= MC_EPS) ? s > 0 : false;
+    case 7: case 11: case 13: case 14: case 15: return s < 0;
+    default: return s > 0;
+  }
+}
+
+// Tiling of one cube: pointer to its table row and the triangle count.
+__device__ inline int mc_tiling(const double* v, int idx, const int8_t** tri) {
+  const int cas = MC_CASES[idx][0], cfg = MC_CASES[idx][1];
+  const int8_t* t = nullptr;
+  int n = 0;
+  switch (cas) {
+    case 0: break;
+    case 1: t = MC_TILING1[cfg]; n = 1; break;
+    case 2: t = MC_TILING2[cfg]; n = 2; break;
+    case 3:
+      if (mc_face(v, MC_TEST3[cfg])) { t = MC_TILING3_2[cfg]; n = 4; }
+      else { t = MC_TILING3_1[cfg]; n = 2; }
+      break;
+    case 4:
+      if (mc_interior(v, 4, cfg, 0, MC_TEST4[cfg])) { t = MC_TILING4_1[cfg]; n = 2; }
+      else { t = MC_TILING4_2[cfg]; n = 6; }
+      break;
+    case 5: t = MC_TILING5[cfg]; n = 3; break;
+    case 6:
+      if (mc_face(v, MC_TEST6[cfg][0])) { t = MC_TILING6_2[cfg]; n = 5; }
+      else if (mc_interior(v, 6, cfg, 0, MC_TEST6[cfg][1])) { t = MC_TILING6_1_1[cfg]; n = 3; }
+      else { t = MC_TILING6_1_2[cfg]; n = 9; }
+      break;
+    case 7: {
+      const int sub = mc_face(v, MC_TEST7[cfg][0]) | (mc_face(v, MC_TEST7[cfg][1]) << 1) |
+                      (mc_face(v, MC_TEST7[cfg][2]) << 2);
+      switch (sub) {
+        case 0: t = MC_TILING7_1[cfg]; n = 3; break;
+        case 1: t = MC_TILING7_2[cfg][0]; n = 5; break;
+        case 2: t = MC_TILING7_2[cfg][1]; n = 5; break;
+        case 3: t = MC_TILING7_3[cfg][0]; n = 9; break;
+        case 4: t = MC_TILING7_2[cfg][2]; n = 5; break;
+        case 5: t = MC_TILING7_3[cfg][1]; n = 9; break;
+        case 6: t = MC_TILING7_3[cfg][2]; n = 9; break;
+        default:
+          if (mc_interior(v, 7, cfg, 0, MC_TEST7[cfg][3])) { t = MC_TILING7_4_2[cfg]; n = 9; }
+          else { t = MC_TILING7_4_1[cfg]; n = 5; }
+      }
+      break;
+    }
+    case 8: t = MC_TILING8[cfg]; n = 2; break;
+    case 9: t = MC_TILING9[cfg]; n = 4; break;
+    case 10:
+      if (mc_face(v, MC_TEST10[cfg][0])) {
+        if (mc_face(v, MC_TEST10[cfg][1])) { t = MC_TILING10_1_1_[cfg]; n = 4; }
+        else { t = MC_TILING10_2[cfg]; n = 8; }
+      } else if (mc_face(v, MC_TEST10[cfg][1])) { t = MC_TILING10_2_[cfg]; n = 8; }
+      else if (mc_interior(v, 10, cfg, 0, MC_TEST10[cfg][2])) { t = MC_TILING10_1_1[cfg]; n = 4; }
+      else { t = MC_TILING10_1_2[cfg]; n = 8; }
+      break;
+    case 11: t = MC_TILING11[cfg]; n = 4; break;
+    case 12:
+      if (mc_face(v, MC_TEST12[cfg][0])) {
+        if (mc_face(v, MC_TEST12[cfg][1])) { t = MC_TILING12_1_1_[cfg]; n = 4; }
+        else { t = MC_TILING12_2[cfg]; n = 8; }
+      } else if (mc_face(v, MC_TEST12[cfg][1])) { t = MC_TILING12_2_[cfg]; n = 8; }
+      else if (mc_interior(v, 12, cfg, 0, MC_TEST12[cfg][2])) { t = MC_TILING12_1_1[cfg]; n = 4; }
+      else { t = MC_TILING12_1_2[cfg]; n = 8; }
+      break;
+    case 13: {
+      int sub = 0;
+      for (int k = 0; k < 6; ++k) sub |= mc_face(v, MC_TEST13[cfg][k]) << k;
+      const int sc = MC_SUBCONFIG13[sub];
+      if (sc == 0) { t = MC_TILING13_1[cfg]; n = 4; }
+      else if (sc <= 6) { t = MC_TILING13_2[cfg][sc - 1]; n = 6; }
+      else if (sc <= 18) { t = MC_TILING13_3[cfg][sc - 7]; n = 10; }
+      else if (sc <= 22) { t = MC_TILING13_4[cfg][sc - 19]; n = 12; }
+      else if (sc <= 26) {
+        if (mc_interior(v, 13, cfg, sc - 23, MC_TEST13[cfg][6])) { t = MC_TILING13_5_1[cfg][sc - 23]; n = 6; }
+        else { t = MC_TILING13_5_2[cfg][sc - 23]; n = 10; }
+      } else if (sc <= 38) { t = MC_TILING13_3_[cfg][sc - 27]; n = 10; }
+      else if (sc <= 44) { t = MC_TILING13_2_[cfg][sc - 39]; n = 6; }
+      else if (sc == 45) { t = MC_TILING13_1_[cfg]; n = 4; }
+      break;
+    }
+    case 14: t = MC_TILING14[cfg]; n = 4; break;
+  }
+  *tri = t;
+  return n;
+}
+
+// Corner values of the cell with lowest corner (x, y, z); returns the case index.
+__device__ inline int mc_load_cell(const McVol& V, int x, int y, int z, double* v, double* raw) {
+  int idx = 0;
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    raw[p] = mc_value(V, x + MC_CX[p], y + MC_CY[p], z + MC_CZ[p]);
+    v[p] = raw[p];
+    idx |= (v[p] > 0) << p;
+  }
+  return idx;
+}
+
+__device__ inline bool mc_uses_centre(const int8_t* t, int n) {
+  for (int k = 0; k < 3 * n; ++k)
+    if (t[k] == 12) return true;
+  return false;
+}
+
+// Per point: owned-edge flags (bit 0 x, 1 y, 2 z) and the counts.
+struct McPoint {
+  int flags, nv, nt;
+};
+
+__device__ inline int mc_edge_flags(const McVol& V, int x, int y, int z) {
+  const bool p0 = mc_pos(mc_value(V, x, y, z));
+  int f = 0;
+  if (x + 1 < V.nx && mc_pos(mc_value(V, x + 1, y, z)) != p0) f |= 1;
+  if (y + 1 < V.ny && mc_pos(mc_value(V, x, y + 1, z)) != p0) f |= 2;
+  if (z + 1 < V.nz && mc_pos(mc_value(V, x, y, z + 1)) != p0) f |= 4;
+  return f;
+}
+
+__device__ inline McPoint mc_point(const McVol& V, int64_t i) {
+  McPoint r{0, 0, 0};
+  const int x = (int)(i % V.nx);
+  const int y = (int)((i / V.nx) % V.ny);
+  const int z = (int)(i / ((int64_t)V.nx * V.ny));
+  r.flags = mc_edge_flags(V, x, y, z);
+  r.nv = __popc(r.flags);
+  if (x + 1 < V.nx && y + 1 < V.ny && z + 1 < V.nz) {
+    double v[8], raw[8];
+    const int idx = mc_load_cell(V, x, y, z, v, raw);
+    if (idx != 0 && idx != 255) {
+      const int8_t* t;
+      r.nt = mc_tiling(v, idx, &t);
+      if (mc_uses_centre(t, r.nt)) r.nv += 1;
+    }
+  }
+  return r;
+}
+
+// Block-wide exclusive scan of two counters (256 threads = 4 waves).
+__device__ inline void block_scan2(int a, int b, int* ea, int* eb, int* ta, int* tb) {
+  __shared__ int sa[MC_BLOCK / 64], sb[MC_BLOCK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int ia = a, ib = b;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int xa = __shfl_up(ia, o), xb = __shfl_up(ib, o);
+    if (lane >= o) { ia += xa; ib += xb; }
+  }
+  if (lane == 63) { sa[w] = ia; sb[w] = ib; }
+  __syncthreads();
+  int oa = 0, ob = 0, sumA = 0, sumB = 0;
+#pragma unroll
+  for (int k = 0; k < MC_BLOCK / 64; ++k) {
+    if (k < w) { oa += sa[k]; ob += sb[k]; }
+    sumA += sa[k];
+    sumB += sb[k];
+  }
+  *ea = oa + ia - a;
+  *eb = ob + ib - b;
+  *ta = sumA;
+  *tb = sumB;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_count(McVol V, int64_t n, uint32_t* block_tot) {
+  const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
+  McPoint p{0, 0, 0};
+  if (i < n) p = mc_point(V, i);
+  int ea, eb, ta, tb;
+  block_scan2(p.nv, p.nt, &ea, &eb, &ta, &tb);
+  if (threadIdx.x == 0) {
+    block_tot[2 * blockIdx.x + 0] = (uint32_t)ta;
+    block_tot[2 * blockIdx.x + 1] = (uint32_t)tb;
+  }
+}
+
+// Exclusive scan of the block sums in place (one workgroup of 1024 threads),
+// totals into tot[0..1] as 64-bit values.
+__global__ __launch_bounds__(1024) void k_mc_scan(uint32_t* block_tot, int64_t nblocks,
+                                                  unsigned long long* tot) {
+  __shared__ unsigned long long s0[1024], s1[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (nblocks + 1023) / 1024;
+  const int64_t lo = t * per, hi = (lo + per < nblocks) ? lo + per : nblocks;
+  unsigned long long a = 0, b = 0;
+  for (int64_t k = lo; k < hi; ++k) { a += block_tot[2 * k]; b += block_tot[2 * k + 1]; }
+  s0[t] = a;
+  s1[t] = b;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    unsigned long long xa = 0, xb = 0;
+    if (t >= o) { xa = s0[t - o]; xb = s1[t - o]; }
+    __syncthreads();
+    s0[t] += xa;
+    s1[t] += xb;
+    __syncthreads();
+  }
+  unsigned long long oa = s0[t] - a, ob = s1[t] - b;
+  for (int64_t k = lo; k < hi; ++k) {
+    const uint32_t ca = block_tot[2 * k], cb = block_tot[2 * k + 1];
+    block_tot[2 * k] = (uint32_t)oa;
+    block_tot[2 * k + 1] = (uint32_t)ob;
+    oa += ca;
+    ob += cb;
+  }
+  if (t == 1023) { tot[0] = s0[1023]; tot[1] = s1[1023]; }
+}
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, int64_t n, const uint32_t* block_off,
+                                                          uint32_t* vbase, float* verts, float* normals,
+                                                          float* values) {
+  const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
+  McPoint p{0, 0, 0};
+  if (i < n) p = mc_point(V, i);
+  int ea, eb, ta, tb;
+  block_scan2(p.nv, 0, &ea, &eb, &ta, &tb);
+  if (i >= n) return;
+  const uint32_t base = block_off[2 * blockIdx.x] + (uint32_t)ea;
+  vbase[i] = base;
+  if (p.nv == 0) return;
+  const int x = (int)(i % V.nx);
+  const int y = (int)((i / V.nx) % V.ny);
+  const int z = (int)(i / ((int64_t)V.nx * V.ny));
+  const double v0 = mc_value(V, x, y, z);
+  uint32_t id = base;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!(p.flags & (1 << ax))) continue;
+    const int x1 = x + (ax == 0), y1 = y + (ax == 1), z1 = z + (ax == 2);
+    const double v1 = mc_value(V, x1, y1, z1);
+    const double w0 = mc_weight(v0), w1 = mc_weight(v1);
+    const double t = w1 / (w0 + w1);
+    double pos[3] = {(double)x, (double)y, (double)z};
+    pos[ax] += t;
+    verts[3 * (int64_t)id + 0] = (float)pos[2];
+    verts[3 * (int64_t)id + 1] = (float)pos[1];
+    verts[3 * (int64_t)id + 2] = (float)pos[0];
+    if (normals) {
+      // -grad of the volume by central differences (one-sided at the border),
+      // interpolated along the edge, unit length
+      double g[3];
+      for (int k = 0; k < 3; ++k) {
+        const int dx = k == 0, dy = k == 1, dz = k == 2;
+        auto cd = [&](int px, int py, int pz) {
+          const int ax0 = px - dx >= 0 ? px - dx : px, ay0 = py - dy >= 0 ? py - dy : py,
+                    az0 = pz - dz >= 0 ? pz - dz : pz;
+          const int ax1 = px + dx < V.nx ? px + dx : px, ay1 = py + dy < V.ny ? py + dy : py,
+                    az1 = pz + dz < V.nz ? pz + dz : pz;
+          const double span = (double)((ax1 - ax0) + (ay1 - ay0) + (az1 - az0));
+          return (mc_value(V, ax1, ay1, az1) - mc_value(V, ax0, ay0, az0)) / (span > 0 ? span : 1.0);
+        };
+        g[k] = (1 - t) * cd(x, y, z) + t * cd(x1, y1, z1);
+      }
+      const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      const double s = len > 0 ? -1.0 / len : 0.0;
+      normals[3 * (int64_t)id + 0] = (float)(g[2] * s);
+      normals[3 * (int64_t)id + 1] = (float)(g[1] * s);
+      normals[3 * (int64_t)id + 2] = (float)(g[0] * s);
+    }
+    if (values) values[id] = (float)fabs(v1 - v0);
+    ++id;
+  }
+  if (id < base + (uint32_t)p.nv) {
+    // centre vertex of the cell at this point
+    double v[8], raw[8];
+    mc_load_cell(V, x, y, z, v, raw);
+    double s[3] = {0, 0, 0}, ws = 0, vmax = -1e300, vmin = 1e300;
+    for (int q = 0; q < 8; ++q) {
+      const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+      const int lp = dz * 4 + (dy ? (dx ? 2 : 3) : (dx ? 1 : 0));
+      const double w = mc_weight(raw[lp]);
+      s[0] += w * dx;
+      s[1] += w * dy;
+      s[2] += w * dz;
+      ws += w;
+      vmax = raw[lp] > vmax ? raw[lp] : vmax;
+      vmin = raw[lp] < vmin ? raw[lp] : vmin;
+    }
+    verts[3 * (int64_t)id + 0] = (float)(z + s[2] / ws);
+    verts[3 * (int64_t)id + 1] = (float)(y + s[1] / ws);
+    verts[3 * (int64_t)id + 2] = (float)(x + s[0] / ws);
+    if (normals) {
+      // gradient of the trilinear interpolant at the cell centre
+      const double gx = (raw[1] + raw[2] + raw[5] + raw[6] - raw[0] - raw[3] - raw[4] - raw[7]) * 0.25;
+      const double gy = (raw[2] + raw[3] + raw[6] + raw[7] - raw[0] - raw[1] - raw[4] - raw[5]) * 0.25;
+      const double gz = (raw[4] + raw[5] + raw[6] + raw[7] - raw[0] - raw[1] - raw[2] - raw[3]) * 0.25;
+      const double len = sqrt(gx * gx + gy * gy + gz * gz);
+      const double sc = len > 0 ? -1.0 / len : 0.0;
+      normals[3 * (int64_t)id + 0] = (float)(gz * sc);
+      normals[3 * (int64_t)id + 1] = (float)(gy * sc);
+      normals[3 * (int64_t)id + 2] = (float)(gx * sc);
+    }
+    if (values) values[id] = (float)(vmax - vmin);
+  }
+}
+
+__device__ inline uint32_t mc_vertex_id(const McVol& V, const uint32_t* vbase, int x, int y, int z, int e) {
+  if (e == 12) return vbase[((int64_t)z * V.ny + y) * V.nx + x] + __popc(mc_edge_flags(V, x, y, z));
+  const int ox = x + MC_OX[e], oy = y + MC_OY[e], oz = z + MC_OZ[e];
+  const int ax = MC_AX[e];
+  const int f = mc_edge_flags(V, ox, oy, oz);
+  return vbase[((int64_t)oz * V.ny + oy) * V.nx + ox] + __popc(f & ((1 << ax) - 1));
+}
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_faces(McVol V, int64_t n, const uint32_t* block_off,
+                                                       const uint32_t* vbase, int32_t* faces) {
+  const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
+  int nt = 0, idx = 0;
+  double v[8], raw[8];
+  int x = 0, y = 0, z = 0;
+  const int8_t* tri = nullptr;
+  if (i < n) {
+    x = (int)(i % V.nx);
+    y = (int)((i / V.nx) % V.ny);
+    z = (int)(i / ((int64_t)V.nx * V.ny));
+    if (x + 1 < V.nx && y + 1 < V.ny && z + 1 < V.nz) {
+      idx = mc_load_cell(V, x, y, z, v, raw);
+      if (idx != 0 && idx != 255) nt = mc_tiling(v, idx, &tri);
+    }
+  }
+  int ea, eb, ta, tb;
+  block_scan2(nt, 0, &ea, &eb, &ta, &tb);
+  if (nt == 0) return;
+  const int64_t fb = (int64_t)block_off[2 * blockIdx.x + 1] + ea;
+  for (int k = 0; k < nt; ++k) {
+    // 'descent': faces in reversed table order
+    for (int j = 0; j < 3; ++j)
+      faces[3 * (fb + k) + j] = (int32_t)mc_vertex_id(V, vbase, x, y, z, tri[3 * k + 2 - j]);
+  }
+}
+
+static size_t mc_blocks(int64_t n) { return (size_t)((n + MC_BLOCK - 1) / MC_BLOCK); }
+
+static size_t mc_ws_bytes(int64_t n) {
+  return 256 + ((mc_blocks(n) * 8 + 255) & ~(size_t)255) + (size_t)n * 4;
+}
+
+static int mc_check(const float* vol, int nz, int ny, int nx, void* ws, const char* fn) {
+  MP_REQUIRE(vol != nullptr && ws != nullptr, MP_EINVAL, "%s: vol and ws must be non-NULL", fn);
+  MP_REQUIRE(nz >= 2 && ny >= 2 && nx >= 2, MP_EINVAL,
+             "%s: the volume must be at least 2x2x2, got %dx%dx%d", fn, nz, ny, nx);
+  MP_REQUIRE((int64_t)nz * ny * nx <= (1ll << 31), MP_EUNSUPPORTED, "%s: volume too large", fn);
+  return MP_OK;
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+extern "C" {
+
+size_t mp_mc_workspace_size(int nz, int ny, int nx) {
+  if (nz < 2 || ny < 2 || nx < 2) return 0;
+  return mc_ws_bytes((int64_t)nz * ny * nx);
+}
+
+int mp_mc_count(const float* vol, int nz, int ny, int nx, float level, int64_t* n_verts,
+                int64_t* n_faces, void* ws, void* stream) {
+  int rc = mc_check(vol, nz, ny, nx, ws, "mp_mc_count");
+  if (rc != MP_OK) return rc;
+  hipStream_t s = as_stream(stream);
+  const int64_t n = (int64_t)nz * ny * nx;
+  McVol V{vol, nx, ny, nz, (double)level};
+  char* w = reinterpret_cast<char*>(ws);
+  unsigned long long* tot = reinterpret_cast<unsigned long long*>(w);
+  uint32_t* block = reinterpret_cast<uint32_t*>(w + 256);
+  const size_t nb = mc_blocks(n);
+  hipLaunchKernelGGL(k_mc_count, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block);
+  MP_LAUNCHED("k_mc_count");
+  hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, block, (int64_t)nb, tot);
+  MP_LAUNCHED("k_mc_scan");
+  unsigned long long h[2] = {0, 0};
+  MP_HIP(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, s));
+  MP_HIP(hipStreamSynchronize(s));
+  MP_REQUIRE(h[0] < (1ull << 31) && h[1] < (1ull << 31), MP_EUNSUPPORTED,
+             "mp_mc_count: %llu vertices / %llu faces exceed int32 indexing", h[0], h[1]);
+  if (n_verts) *n_verts = (int64_t)h[0];
+  if (n_faces) *n_faces = (int64_t)h[1];
+  return MP_OK;
+}
+
+int mp_mc_emit(const float* vol, int nz, int ny, int nx, float level, float* verts, int32_t* faces,
+               float* normals, float* values, void* ws, void* stream) {
+  int rc = mc_check(vol, nz, ny, nx, ws, "mp_mc_emit");
+  if (rc != MP_OK) return rc;
+  // verts / faces may be NULL only when mp_mc_count returned zero counts
+  hipStream_t s = as_stream(stream);
+  const int64_t n = (int64_t)nz * ny * nx;
+  McVol V{vol, nx, ny, nz, (double)level};
+  char* w = reinterpret_cast<char*>(ws);
+  const size_t nb = mc_blocks(n);
+  const uint32_t* block = reinterpret_cast<const uint32_t*>(w + 256);
+  uint32_t* vbase = reinterpret_cast<uint32_t*>(w + 256 + ((nb * 8 + 255) & ~(size_t)255));
+  hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, vbase,
+                     verts, normals, values);
+  MP_LAUNCHED("k_mc_vertices");
+  hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, vbase, faces);
+  MP_LAUNCHED("k_mc_faces");
+  return MP_OK;
+}
+
+}  // extern "C"

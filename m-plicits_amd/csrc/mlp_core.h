@@ -22,6 +22,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifndef MP_PF
+#define MP_PF 4
+#endif
+
 namespace mp {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -40,6 +44,7 @@ enum ShapeId { SH_32x32x16_BF16 = 0, SH_32x32x16_F16 = 1, SH_16x16x32_BF16 = 2,
                SH_16x16x32_F16 = 3, SH_16x16x4_F32 = 4, SH_32x32x2_F32 = 5 };
 
 struct S32x32x16_bf16 {
+  static constexpr bool FIRST_IN_REVS = true;
   static constexpr int ROWS = 32, KB = 16, EPL = 8, CREGS = 16, GROUPS = 2;
   typedef bf16x8 Frag;
   typedef f32x16 Acc;
@@ -51,6 +56,7 @@ struct S32x32x16_bf16 {
 };
 
 struct S32x32x16_f16 {
+  static constexpr bool FIRST_IN_REVS = true;
   static constexpr int ROWS = 32, KB = 16, EPL = 8, CREGS = 16, GROUPS = 2;
   typedef f16x8 Frag;
   typedef f32x16 Acc;
@@ -62,6 +68,7 @@ struct S32x32x16_f16 {
 };
 
 struct S16x16x32_bf16 {
+  static constexpr bool FIRST_IN_REVS = true;
   static constexpr int ROWS = 16, KB = 32, EPL = 8, CREGS = 4, GROUPS = 4;
   typedef bf16x8 Frag;
   typedef f32x4 Acc;
@@ -73,6 +80,7 @@ struct S16x16x32_bf16 {
 };
 
 struct S16x16x32_f16 {
+  static constexpr bool FIRST_IN_REVS = true;
   static constexpr int ROWS = 16, KB = 32, EPL = 8, CREGS = 4, GROUPS = 4;
   typedef f16x8 Frag;
   typedef f32x4 Acc;
@@ -84,6 +92,7 @@ struct S16x16x32_f16 {
 };
 
 struct S16x16x4_f32 {
+  static constexpr bool FIRST_IN_REVS = false;
   static constexpr int ROWS = 16, KB = 4, EPL = 1, CREGS = 4, GROUPS = 4;
   typedef float Frag;
   typedef f32x4 Acc;
@@ -95,6 +104,7 @@ struct S16x16x4_f32 {
 };
 
 struct S32x32x2_f32 {
+  static constexpr bool FIRST_IN_REVS = false;
   static constexpr int ROWS = 32, KB = 2, EPL = 1, CREGS = 16, GROUPS = 2;
   typedef float Frag;
   typedef f32x16 Acc;
@@ -129,12 +139,16 @@ __host__ __device__ inline int shape_perm(const ShapeInfo& s, int v) {
 }
 
 // ---------------------------------------------------------------------------
-// Sine.  v_sin_f32 takes its argument in revolutions.  The first layer sees
-// arguments of tens of radians (omega_0 = 30 baked in), so it is reduced first
-// with a two-constant Cody-Waite step: r = a - k*2pi, 2pi = HI + LO,
-// HI = fl32(2pi), LO = 2pi - HI.
+// Sine.  v_sin_f32 takes its argument in revolutions.  The packer scales the
+// hidden layers' weights and biases by 1/(2 pi), so their accumulators are
+// already in revolutions (sin_rev).  The first layer sees arguments of tens of
+// radians (omega_0 = 30 baked in): in the FP32 path it stays in radians and is
+// reduced with a two-constant Cody-Waite step, r = a - k*2pi with
+// 2pi = HI + LO, HI = fl32(2pi), LO = 2pi - HI; in the 16-bit paths its
+// weights are scaled to revolutions too (accuracy of the FP32 reference's own
+// rounding of a at |a| ~ 50).
 // ---------------------------------------------------------------------------
-__device__ inline float sin_rev(float a) { return __builtin_amdgcn_sinf(a * 0.159154943091895336f); }
+__device__ inline float sin_rev(float r) { return __builtin_amdgcn_sinf(r); }
 
 __device__ inline float sin_reduced(float a) {
   const float k = rintf(a * 0.159154943091895336f);
@@ -184,7 +198,7 @@ struct Mlp {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const float a = fmaf(w.z, x[c][2], fmaf(w.y, x[c][1], fmaf(w.x, x[c][0], w.w)));
-          S::set(b[c][kb], e, sin_reduced(a));
+          S::set(b[c][kb], e, S::FIRST_IN_REVS ? sin_rev(a) : sin_reduced(a));
         }
       }
     }
@@ -194,7 +208,7 @@ struct Mlp {
   // fragments ahead of their MFMA; the empty asm with a memory clobber stops
   // the compiler from hoisting the whole row block's loads (which would cost
   // KBS * 4 VGPRs and spill the activations).
-  static constexpr int PF = 4;
+  static constexpr int PF = MP_PF;
 
   // One weight layer: for every row block rb, acc = b + W . bin, then
   // epi(rb, acc) consumes the FP32 accumulators.
@@ -206,21 +220,19 @@ struct Mlp {
     for (int d = 0; d < PF; ++d) pf[d] = A[d * 64 + lane];
 #pragma unroll
     for (int rb = 0; rb < RBS; ++rb) {
-      Acc acc[NC];
+      // the bias enters as the C operand of the first MFMA of the row block
+      Acc bias;
       const float* brow = bc + (rb * S::GROUPS + g) * S::CREGS;
 #pragma unroll
-      for (int q = 0; q < S::CREGS; ++q) {
-        const float bv = brow[q];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c][q] = bv;
-      }
+      for (int q = 0; q < S::CREGS; ++q) bias[q] = brow[q];
+      Acc acc[NC];
 #pragma unroll
       for (int kb = 0; kb < KBS; ++kb) {
         const int idx = rb * KBS + kb;
         const Frag a = pf[idx % PF];
         if (idx + PF < RBS * KBS) pf[idx % PF] = A[(idx + PF) * 64 + lane];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] = S::mma(a, bin[c][kb], acc[c]);
+        for (int c = 0; c < NC; ++c) acc[c] = S::mma(a, bin[c][kb], kb == 0 ? bias : acc[c]);
         asm volatile("" ::: "memory");
       }
       epi(rb, acc);
@@ -285,7 +297,7 @@ struct Mlp {
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const float a = fmaf(w.z, x[c][2], fmaf(w.y, x[c][1], fmaf(w.x, x[c][0], w.w)));
-            f[c] = fmaf(wl, sin_reduced(a), f[c]);
+            f[c] = fmaf(wl, S::FIRST_IN_REVS ? sin_rev(a) : sin_reduced(a), f[c]);
           }
         }
       }

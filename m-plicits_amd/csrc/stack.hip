@@ -29,10 +29,13 @@ void set_error(const char* fmt, ...) {
 const char* get_error() { return g_error.c_str(); }
 
 int default_nc(int shape, int width) {
-  (void)shape;
-  (void)width;
-  return 1;
+  const ShapeInfo s = shape_info(shape);
+  return (s.elem_bytes == 2 && width <= 256) ? 2 : 1;
 }
+
+// 1 / (2 pi): hidden layers are packed in revolutions for v_sin_f32.
+static constexpr float INV_2PI = 0.159154943091895336f;
+__host__ __device__ inline bool first_in_revs(int shape) { return shape_info(shape).elem_bytes == 2; }
 
 int ensure_scratch(mp_stack* st, size_t bytes) {
   if (st->scratch_bytes >= bytes) return MP_OK;
@@ -85,6 +88,7 @@ __global__ void k_pack_small(PackArgs a) {
     if (u < a.dims[1]) {
       const float* row = a.W + a.woff[0] + (int64_t)u * 3;
       w = make_float4(row[0], row[1], row[2], a.B[a.boff[0] + u]);
+      if (first_in_revs(a.shape)) w = make_float4(w.x * INV_2PI, w.y * INV_2PI, w.z * INV_2PI, w.w * INV_2PI);
     }
     a.w0v[i] = w;
     // output layer, virtual order
@@ -94,7 +98,8 @@ __global__ void k_pack_small(PackArgs a) {
     const int r = rb * s.rows + shape_rowmap(s, g, q);
     a.wlc[i] = (r < a.dims[H]) ? a.W[a.woff[H] + r] : 0.f;
     // hidden biases, C order
-    for (int j = 1; j < H; ++j) a.biasc[(int64_t)(j - 1) * a.width + i] = (r < a.dims[j + 1]) ? a.B[a.boff[j] + r] : 0.f;
+    for (int j = 1; j < H; ++j)
+      a.biasc[(int64_t)(j - 1) * a.width + i] = (r < a.dims[j + 1]) ? a.B[a.boff[j] + r] * INV_2PI : 0.f;
   }
   if (i == 0) *a.blp = a.B[a.boff[H]];
 }
@@ -114,7 +119,7 @@ __global__ void k_pack_hidden(PackArgs a, int j) {
   const int v = kb * s.kb + (lane / s.rows) * s.epl + e;
   const int u = shape_perm(s, v);
   const int out_j = a.dims[j + 1], in_j = a.dims[j];
-  const float w = (r < out_j && u < in_j) ? a.W[a.woff[j] + (int64_t)r * in_j + u] : 0.f;
+  const float w = (r < out_j && u < in_j) ? a.W[a.woff[j] + (int64_t)r * in_j + u] * INV_2PI : 0.f;
   const int64_t o = (int64_t)(j - 1) * per_layer + i;
   switch (a.shape) {
     case SH_32x32x16_BF16: case SH_16x16x32_BF16: reinterpret_cast<__bf16*>(a.apk)[o] = (__bf16)w; break;
@@ -256,6 +261,14 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     return rc;
   }
   *out = st;
+  return MP_OK;
+}
+
+int mp_stack_set_tiles(mp_stack* st, int column_blocks) {
+  MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_stack_set_tiles: stack is NULL");
+  MP_REQUIRE(column_blocks == 1 || column_blocks == 2, MP_EINVAL,
+             "mp_stack_set_tiles: column_blocks must be 1 or 2, got %d", column_blocks);
+  st->nc = column_blocks;
   return MP_OK;
 }
 

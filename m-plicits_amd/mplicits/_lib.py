@@ -53,6 +53,7 @@ SIGNATURES = {
     "mp_stack_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "mp_stack_destroy": (None, [ctypes.c_void_p]),
     "mp_stack_levels": (ctypes.c_int, [ctypes.c_void_p]),
+    "mp_stack_set_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mp_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mp_extract_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
@@ -60,6 +61,13 @@ SIGNATURES = {
                                          ctypes.c_void_p]),
     "mp_grid_coords": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_mc_workspace_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "mp_mc_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, c_int64_p, c_int64_p, ctypes.c_void_p,
+                                   ctypes.c_void_p]),
+    "mp_mc_emit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

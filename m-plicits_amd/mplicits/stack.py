@@ -90,6 +90,10 @@ class LevelStack:
     def n_levels(self):
         return len(self.models)
 
+    def set_tiles(self, column_blocks):
+        """Points per wave = column_blocks x the MFMA block (the autotuner's knob)."""
+        _lib.check(self._lib.mp_stack_set_tiles(self._handle, int(column_blocks)))
+
     def _current_versions(self):
         return tuple((w._version, b._version, w.data_ptr(), b.data_ptr())
                      for layers in self._layers for w, b, _ in layers)

@@ -69,10 +69,18 @@ def test_chunk_break_semantics():
 
 
 def test_reduced_precision_volume_close():
+    # Reduced precision moves values by ~1e-4, so a sample sitting on a band
+    # edge may be culled differently; compare the unculled volumes and the band
+    # counts.
     levels = _levels()
     N, chunk = 48, 32 ** 3
-    ref = _oracle_volume(levels, N, [0.05, 0.02], chunk)
+    ref = _oracle_volume(levels, N, [], chunk)
+    _, counts32 = LevelStack(levels, "fp32").extract_volume(N, [0.05, 0.02], chunk=chunk)
     for prec, tol in (("fp16", 2e-3), ("bf16", 1e-2)):
-        vol, _ = LevelStack(levels, prec).extract_volume(N, [0.05, 0.02], chunk=chunk)
+        st = LevelStack(levels, prec)
+        vol, _ = st.extract_volume(N, [], chunk=chunk)
         err = np.abs(vol.double().cpu().numpy().reshape(-1) - ref).max()
         assert err <= tol * np.abs(ref).max(), (prec, err)
+        _, counts = st.extract_volume(N, [0.05, 0.02], chunk=chunk)
+        for a, b in zip(counts, counts32):
+            assert abs(a - b) <= 0.01 * b + 8, (prec, counts, counts32)
