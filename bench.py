@@ -152,7 +152,20 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precisions", default="bf16,fp16,fp32",
+                    help="comma list of paths to time (profiling runs use one)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB tensor (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
@@ -162,7 +175,7 @@ def main():
     vol = torch.empty(N, N, N, device="cuda")
 
     results = {}
-    for prec in ("bf16", "fp16", "fp32"):
+    for prec in args.precisions.split(","):
         st = LevelStack(levels, prec)
         steps = args.steps if prec != "fp32" else max(1, min(args.steps, 2))
         ms, counts, nv, nf = timed(st, vol, steps, min(args.warmup, 1) if prec == "fp32" else args.warmup)
@@ -190,7 +203,8 @@ def main():
                          f"restatement of the reference sampling loop, scaled x{4096 / len(ids):.1f}; "
                          "marching cubes excluded"}
 
-    b = results["bf16"]
+    b = results.get("bf16") or next(iter(results.values()))
+    fp32 = results.get("fp32")
     out = {
         "metric": "512^3 band-culled extraction ms, rays/s, train pts/s: BF16 vs FP32 at SDF err",
         "value": round(b["ms"], 3), "unit": "ms per 512^3 band-culled extraction (BF16, incl. GPU MC)",
@@ -200,9 +214,8 @@ def main():
         "config": {"workload": "three-level band-culled extraction, 512^3 grid, 32^3 chunks",
                    "levels": "256x3 w30 > 256x3 w60 (delta 0.05) > 512x3 w90 (delta 0.02)",
                    "grid": "reference", "global_batch": N ** 3, "parallelism": "none (1 GPU)"},
-        "fp32_ms": round(results["fp32"]["ms"], 3),
-        "fp16_ms": round(results["fp16"]["ms"], 3),
-        "speedup_bf16_vs_fp32": round(results["fp32"]["ms"] / b["ms"], 2),
+        "ms_by_precision": {p: round(r["ms"], 3) for p, r in results.items()},
+        "speedup_bf16_vs_fp32": round(fp32["ms"] / b["ms"], 2) if fp32 else None,
         "level_counts": b["counts"], "mesh": {"verts": b["verts"], "faces": b["faces"]},
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
         "roofline": roofline,

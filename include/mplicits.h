@@ -118,6 +118,35 @@ int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, v
  * verts and faces may be NULL only when the counts are zero.
  * `ws` is a device workspace of mp_mc_workspace_size(nz, ny, nx) bytes, kept
  * between the two calls. */
+/* Multiscale sphere tracing of the whole stack.  Replaces the renderer's
+ * MIPplicit::SdfInference tracing loop and shading (renderer/cuda/state.cu:816-1055;
+ * InitializeRays_kernel, IterateSphereTracing_kernel and Shade_kernel in
+ * renderer/cuda/kernels.cu:122-487) and the Python multiscale_sphere_tracing
+ * (renderer/python/siren-mips.py:126-248).
+ * cams: device [n_cams][32] floats, per camera the inverse view matrix then the
+ * inverse projection matrix, each column-major 4x4 (glm layout, as the renderer
+ * uploads them).  Pixel (i, j) of camera k is ray k*W*H + j*W + i.
+ * Outputs (device, one entry per ray): depth (distance from the camera, -1 for a
+ * miss), steps, rgba (RGBA8: Lambert shading of the analytic normal, white
+ * background), normals [rays][3] (unit, zero for misses; may be NULL).
+ * stats (host, MP_MAX_LEVELS + 1 entries, may be NULL): SDF evaluations per
+ * level, then the number of hits.  Host-sync point.  Allocates device scratch. */
+typedef struct mp_trace_params {
+  int width, height;
+  int max_steps;       /* e.g. 128 */
+  float hit_eps;       /* |F| below this is a hit, e.g. 1e-4 */
+  float bound_radius;  /* rays start on / end outside this sphere (1.2 in the renderer) */
+  float band_scale;    /* residual levels are used where F_0 < band_scale * delta_0 (1.4) */
+  int pixel_center;    /* 1: NDC from pixel centres; 0: the renderer's 2i/W - 1 */
+  int flip_y;
+  int n_deltas;        /* n_levels - 1 */
+  float deltas[MP_MAX_LEVELS];
+} mp_trace_params;
+
+int mp_trace(mp_stack* stack, const float* cams, int n_cams, const mp_trace_params* params,
+             float* depth, float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats,
+             void* stream);
+
 size_t mp_mc_workspace_size(int nz, int ny, int nx);
 int mp_mc_count(const float* vol, int nz, int ny, int nx, float level, int64_t* n_verts,
                 int64_t* n_faces, void* ws, void* stream);

@@ -112,13 +112,3 @@ int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t
 
 }  // namespace mp
 
-namespace mp {
-
-int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
-                hipStream_t stream) {
-  (void)st; (void)lvl0; (void)lvl1; (void)xyz; (void)n; (void)grad; (void)stream;
-  set_error("input gradients are not built yet");
-  return MP_EUNSUPPORTED;
-}
-
-}  // namespace mp

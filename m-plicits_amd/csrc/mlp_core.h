@@ -157,6 +157,8 @@ __device__ inline float sin_reduced(float a) {
   return __builtin_amdgcn_sinf(r * 0.159154943091895336f);
 }
 
+__device__ inline float cos_rev(float r) { return __builtin_amdgcn_cosf(r); }
+
 __device__ inline float cos_reduced(float a) {
   const float k = rintf(a * 0.159154943091895336f);
   float r = fmaf(k, -6.28318548202514648f, a);
@@ -172,6 +174,7 @@ struct LevelDev {
   int n_hidden;         // sine layers
   const float4* w0v;    // [width]  (w_x, w_y, w_z, b) of layer 0, virtual order
   const void* apk;      // [n_hidden-1][width/ROWS][width/KB][64] A fragments
+  const void* apkt;     // same for the transposed hidden weights (backward pass, radians)
   const float* biasc;   // [n_hidden-1][width/ROWS][GROUPS][CREGS] hidden biases, C order
   const float* wlc;     // [width/ROWS][GROUPS][CREGS] output weights, C order
   const float* wlv;     // [width] output weights, virtual order (n_hidden == 1)
@@ -210,8 +213,9 @@ struct Mlp {
   // KBS * 4 VGPRs and spill the activations).
   static constexpr int PF = MP_PF;
 
-  // One weight layer: for every row block rb, acc = b + W . bin, then
-  // epi(rb, acc) consumes the FP32 accumulators.
+  // One weight layer: for every row block rb, acc = b + W . bin (b = 0 when
+  // bc is null: the transposed GEMMs of the backward pass), then epi(rb, acc)
+  // consumes the FP32 accumulators.
   template <class Epi>
   static __device__ inline void gemm(const Frag* __restrict__ A, const float* __restrict__ bc,
                                      const Frag (&bin)[NC][KBS], int lane, int g, Epi epi) {
@@ -222,9 +226,14 @@ struct Mlp {
     for (int rb = 0; rb < RBS; ++rb) {
       // the bias enters as the C operand of the first MFMA of the row block
       Acc bias;
-      const float* brow = bc + (rb * S::GROUPS + g) * S::CREGS;
+      if (bc) {
+        const float* brow = bc + (rb * S::GROUPS + g) * S::CREGS;
 #pragma unroll
-      for (int q = 0; q < S::CREGS; ++q) bias[q] = brow[q];
+        for (int q = 0; q < S::CREGS; ++q) bias[q] = brow[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < S::CREGS; ++q) bias[q] = 0.f;
+      }
       Acc acc[NC];
 #pragma unroll
       for (int kb = 0; kb < KBS; ++kb) {

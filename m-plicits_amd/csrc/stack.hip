@@ -71,6 +71,7 @@ struct PackArgs {
   int64_t boff[17];                  // offset of b_k in B
   float4* w0v;
   void* apk;
+  void* apkt;
   float* biasc;
   float* wlc;
   float* wlv;
@@ -105,7 +106,9 @@ __global__ void k_pack_small(PackArgs a) {
 }
 
 // Hidden layer j (1..H-1): element ((rb * KBS + kb) * 64 + lane) * EPL + e.
-__global__ void k_pack_hidden(PackArgs a, int j) {
+// transpose = 0: A = W_j / (2 pi) (forward, revolutions), rows = layer-j units.
+// transpose = 1: A = W_j^T (backward, radians), rows = layer-(j-1) units.
+__global__ void k_pack_hidden(PackArgs a, int j, int transpose) {
   const ShapeInfo s = shape_info(a.shape);
   const int64_t per_layer = (int64_t)a.width * a.width;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,12 +122,15 @@ __global__ void k_pack_hidden(PackArgs a, int j) {
   const int v = kb * s.kb + (lane / s.rows) * s.epl + e;
   const int u = shape_perm(s, v);
   const int out_j = a.dims[j + 1], in_j = a.dims[j];
-  const float w = (r < out_j && u < in_j) ? a.W[a.woff[j] + (int64_t)r * in_j + u] * INV_2PI : 0.f;
+  float w;
+  if (!transpose) w = (r < out_j && u < in_j) ? a.W[a.woff[j] + (int64_t)r * in_j + u] * INV_2PI : 0.f;
+  else w = (u < out_j && r < in_j) ? a.W[a.woff[j] + (int64_t)u * in_j + r] : 0.f;
   const int64_t o = (int64_t)(j - 1) * per_layer + i;
+  void* dst = transpose ? a.apkt : a.apk;
   switch (a.shape) {
-    case SH_32x32x16_BF16: case SH_16x16x32_BF16: reinterpret_cast<__bf16*>(a.apk)[o] = (__bf16)w; break;
-    case SH_32x32x16_F16: case SH_16x16x32_F16: reinterpret_cast<_Float16*>(a.apk)[o] = (_Float16)w; break;
-    default: reinterpret_cast<float*>(a.apk)[o] = w; break;
+    case SH_32x32x16_BF16: case SH_16x16x32_BF16: reinterpret_cast<__bf16*>(dst)[o] = (__bf16)w; break;
+    case SH_32x32x16_F16: case SH_16x16x32_F16: reinterpret_cast<_Float16*>(dst)[o] = (_Float16)w; break;
+    default: reinterpret_cast<float*>(dst)[o] = w; break;
   }
 }
 
@@ -147,6 +153,7 @@ static int pack_level(const mp_stack* st, const mp_stack::Level& L, hipStream_t 
   }
   a.w0v = const_cast<float4*>(L.dev.w0v);
   a.apk = const_cast<void*>(L.dev.apk);
+  a.apkt = const_cast<void*>(L.dev.apkt);
   a.biasc = const_cast<float*>(L.dev.biasc);
   a.wlc = const_cast<float*>(L.dev.wlc);
   a.wlv = const_cast<float*>(L.dev.wlv);
@@ -155,9 +162,11 @@ static int pack_level(const mp_stack* st, const mp_stack::Level& L, hipStream_t 
   MP_LAUNCHED("k_pack_small");
   const int64_t per_layer = (int64_t)L.width * L.width;
   for (int j = 1; j < L.n_hidden; ++j) {
-    hipLaunchKernelGGL(k_pack_hidden, dim3((unsigned)((per_layer + 255) / 256)), dim3(256), 0,
-                       stream, a, j);
-    MP_LAUNCHED("k_pack_hidden");
+    for (int tr = 0; tr < 2; ++tr) {
+      hipLaunchKernelGGL(k_pack_hidden, dim3((unsigned)((per_layer + 255) / 256)), dim3(256), 0,
+                         stream, a, j, tr);
+      MP_LAUNCHED("k_pack_hidden");
+    }
   }
   return MP_OK;
 }
@@ -236,7 +245,7 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     const size_t sz_bias = n_gemm * W * sizeof(float);
     const size_t sz_wl = W * sizeof(float);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t total = al(sz_w0v) + al(sz_apk) + al(sz_bias) + 2 * al(sz_wl) + 256;
+    const size_t total = al(sz_w0v) + 2 * al(sz_apk) + al(sz_bias) + 2 * al(sz_wl) + 256;
     hipError_t e = hipMalloc(&L.buf, total);
     if (e != hipSuccess) {
       mp::set_error("mp_stack_create: hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
@@ -249,12 +258,16 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     L.dev.n_hidden = L.n_hidden;
     L.dev.w0v = reinterpret_cast<const float4*>(p); p += al(sz_w0v);
     L.dev.apk = p; p += al(sz_apk);
+    L.dev.apkt = p; p += al(sz_apk);
     L.dev.biasc = reinterpret_cast<const float*>(p); p += al(sz_bias);
     L.dev.wlc = reinterpret_cast<const float*>(p); p += al(sz_wl);
     L.dev.wlv = reinterpret_cast<const float*>(p); p += al(sz_wl);
     L.dev.blp = reinterpret_cast<const float*>(p);
   }
-  st->nc = default_nc(st->shape, maxw);
+  // two column blocks per wave for the 16-bit shapes; launch_level drops to
+  // one for levels wider than 256 (their activations would not fit the VGPRs)
+  st->nc = default_nc(st->shape, 0);
+  (void)maxw;
   int rc = mp_stack_update(st, stream);
   if (rc != MP_OK) {
     mp_stack_destroy(st);

@@ -1,0 +1,386 @@
+// Multiscale sphere tracing with a persistent, self-compacting ray queue.
+//
+// Replaces the renderer's graph-captured tracing loop (MIPplicit::SdfInference,
+// renderer/cuda/state.cu:848-881: a fixed number of steps on every pixel, the
+// SIREN GEMM chain of each level on all pixels every step) and
+// IterateSphereTracing_kernel (renderer/cuda/kernels.cu:315-398).  Each wave
+// holds a tile of rays (one per MFMA column); after every step the rays that
+// hit or left the scene are written out and their slots refilled from a global
+// ray counter, so the matrix cores only ever see live rays.  The coarse level
+// is evaluated for every live ray; a residual level only when some ray of the
+// tile is inside the band of the partial sum before it.
+//
+// Step rule (the renderer's, with a hit test):
+//   F_0 = f_0(p).  Outside the coarse band (F_0 >= s * delta_0, s = band_scale,
+//   1.4 in the renderer, kernels.cu:335,374) step by F_0 - delta_0.  Inside it,
+//   F_i = F_{i-1} + f_i(p) for every residual level i whose band the partial sum
+//   is in (|F_{i-1}| < s * delta_{i-1}) and step by the full sum.  A ray hits
+//   when |F| < hit_eps for the evaluated sum, and misses when it leaves the
+//   bounding sphere or runs out of steps.
+// Rays start where they enter the sphere of radius bound_radius
+// (InitializeRays_kernel, kernels.cu:400-487).
+#include "eval_dispatch.h"
+#include "trace.h"
+
+namespace mp {
+
+__global__ void k_fill_ull(unsigned long long* p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <class S, int NC>
+__device__ inline void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC]) {
+  switch (L.width) {
+    case 64: Mlp<S, 64, NC>::eval(L, x, f); break;
+    case 128: Mlp<S, 128, NC>::eval(L, x, f); break;
+    case 256: Mlp<S, 256, NC>::eval(L, x, f); break;
+    default:
+      if constexpr (NC == 1) Mlp<S, 512, 1>::eval(L, x, f);
+      break;
+  }
+}
+
+__device__ inline void ray_init(const TraceArgs& a, int64_t rid, float o[3], float d[3], float& t0,
+                                bool& inside) {
+  const int64_t per_cam = (int64_t)a.width * a.height;
+  const int cam = (int)(rid / per_cam);
+  const int64_t pix = rid % per_cam;
+  const int i = (int)(pix % a.width), j = (int)(pix / a.width);
+  const float* iv = a.cams + cam * 32;
+  const float* ip = iv + 16;
+  const float off = a.pixel_center ? 0.5f : 0.0f;
+  const float nx = (2.0f * (i + off)) / a.width - 1.0f;
+  float ny = (2.0f * (j + off)) / a.height - 1.0f;
+  if (a.flip_y) ny = -ny;
+  const float clip[4] = {nx, ny, -1.0f, 1.0f};
+  float eye[4];
+  for (int k = 0; k < 4; ++k)
+    eye[k] = ip[0 * 4 + k] * clip[0] + ip[1 * 4 + k] * clip[1] + ip[2 * 4 + k] * clip[2] + ip[3 * 4 + k] * clip[3];
+  float world[3];
+  for (int k = 0; k < 3; ++k)
+    world[k] = iv[0 * 4 + k] * eye[0] + iv[1 * 4 + k] * eye[1] + iv[2 * 4 + k] * eye[2] + iv[12 + k];
+  o[0] = iv[12]; o[1] = iv[13]; o[2] = iv[14];
+  d[0] = world[0] - o[0]; d[1] = world[1] - o[1]; d[2] = world[2] - o[2];
+  const float inv = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  d[0] *= inv; d[1] *= inv; d[2] *= inv;
+  // advance to the bounding sphere
+  const float R = a.bound_radius;
+  const float b = o[0] * d[0] + o[1] * d[1] + o[2] * d[2];
+  const float c = o[0] * o[0] + o[1] * o[1] + o[2] * o[2] - R * R;
+  t0 = 0.f;
+  inside = true;
+  if (c > 0.f) {
+    const float disc = b * b - c;
+    if (b < 0.f && disc > 0.f) {
+      t0 = -b - sqrtf(disc);
+      o[0] += t0 * d[0]; o[1] += t0 * d[1]; o[2] += t0 * d[2];
+    } else {
+      inside = false;  // the ray never meets the bounding sphere
+    }
+  }
+}
+
+template <class S, int NC>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane % S::ROWS;
+  const int64_t total = a.total_rays;
+  int64_t rid[NC];
+  float p[NC][3], d[NC][3], t[NC], t0[NC];
+  int steps[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) rid[c] = -1;
+  const float R2 = a.bound_radius * a.bound_radius * 1.0001f;
+  unsigned long long evals[MP_MAX_LEVELS] = {0, 0, 0, 0};
+
+  while (true) {
+    // refill empty slots from the global queue (group-0 lanes vote)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const bool empty = rid[c] < 0;
+      const uint64_t m = __ballot(lane < S::ROWS && empty);
+      if (m == 0) continue;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(a.next_ray, (unsigned long long)__popcll(m));
+      base = __shfl(base, 0);
+      if (lane < S::ROWS && empty) {
+        const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
+        int64_t nid = (int64_t)(base + __popcll(below));
+        while (nid < total) {
+          bool inside;
+          ray_init(a, nid, p[c], d[c], t0[c], inside);
+          if (inside) break;
+          // the ray never meets the bounding sphere: a miss, then the next ray
+          a.depth[nid] = -1.0f;
+          a.steps[nid] = 0;
+          nid = (int64_t)atomicAdd(a.next_ray, 1ull);
+        }
+        rid[c] = nid < total ? nid : -1;
+        t[c] = 0.f;
+        steps[c] = 0;
+      }
+    }
+    // group-1 lanes copy the slot state of their group-0 partner so that both
+    // groups evaluate the same points (the divergent refill above may differ)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int src = col;  // lane in group 0
+      rid[c] = (int64_t)__shfl((long long)rid[c], src);
+      p[c][0] = __shfl(p[c][0], src); p[c][1] = __shfl(p[c][1], src); p[c][2] = __shfl(p[c][2], src);
+      d[c][0] = __shfl(d[c][0], src); d[c][1] = __shfl(d[c][1], src); d[c][2] = __shfl(d[c][2], src);
+      t[c] = __shfl(t[c], src); t0[c] = __shfl(t0[c], src);
+      steps[c] = __shfl(steps[c], src);
+    }
+    bool live = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) live |= rid[c] >= 0;
+    if (__ballot(live) == 0) break;
+
+    float x[NC][3], F[NC], Fsum[NC];
+    bool need[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const bool act = rid[c] >= 0;
+      x[c][0] = act ? p[c][0] : 0.f;
+      x[c][1] = act ? p[c][1] : 0.f;
+      x[c][2] = act ? p[c][2] : 0.f;
+    }
+    eval_level_any<S, NC>(a.levels[0], x, F);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      Fsum[c] = F[c];
+      need[c] = rid[c] >= 0 && a.n_levels > 1 && F[c] < a.band_scale * a.deltas[0];
+      if (lane < S::ROWS && rid[c] >= 0) evals[0]++;
+    }
+    for (int l = 1; l < a.n_levels; ++l) {
+      bool any = false;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) any |= need[c];
+      if (__ballot(any) == 0) break;
+      float r[NC];
+      eval_level_any<S, NC>(a.levels[l], x, r);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (need[c]) {
+          Fsum[c] += r[c];
+          if (lane < S::ROWS) evals[l]++;
+        }
+        need[c] = need[c] && l + 1 < a.n_levels && fabsf(Fsum[c]) < a.band_scale * a.deltas[l];
+      }
+    }
+    // step, hit and exit tests
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (rid[c] < 0) continue;
+      const bool banded = a.n_levels > 1 && F[c] < a.band_scale * a.deltas[0];
+      float s;
+      bool hit;
+      if (a.n_levels == 1) {
+        s = F[c];
+        hit = fabsf(F[c]) < a.hit_eps;
+      } else if (!banded) {
+        s = F[c] - a.deltas[0];
+        hit = false;
+      } else {
+        s = Fsum[c];
+        hit = fabsf(Fsum[c]) < a.hit_eps;
+      }
+      bool done = hit;
+      if (!hit) {
+        p[c][0] += s * d[c][0];
+        p[c][1] += s * d[c][1];
+        p[c][2] += s * d[c][2];
+        t[c] += s;
+        steps[c] += 1;
+        const float r2 = p[c][0] * p[c][0] + p[c][1] * p[c][1] + p[c][2] * p[c][2];
+        // steps are signed: the ray left when it moves away from the centre
+        const float outward = (p[c][0] * d[c][0] + p[c][1] * d[c][1] + p[c][2] * d[c][2]) * s;
+        done = steps[c] >= a.max_steps || (r2 > R2 && outward > 0.f);
+      }
+      if (done) {
+        if (lane < S::ROWS) {
+          const int64_t id = rid[c];
+          a.steps[id] = steps[c];
+          if (hit) {
+            a.depth[id] = t0[c] + t[c];
+            a.hit_pos[3 * id + 0] = p[c][0];
+            a.hit_pos[3 * id + 1] = p[c][1];
+            a.hit_pos[3 * id + 2] = p[c][2];
+          } else {
+            a.depth[id] = -1.0f;
+          }
+        }
+        rid[c] = -1;
+      }
+    }
+  }
+  // per-level evaluation counts (one atomic per lane that counted)
+  if (lane < S::ROWS)
+    for (int l = 0; l < a.n_levels; ++l)
+      if (evals[l]) atomicAdd(&a.stats[l], evals[l]);
+}
+
+// Compact the hit rays: ids and positions.
+__global__ void k_collect_hits(const float* depth, const float* hit_pos, int64_t n, uint32_t* ids,
+                               float* pos, unsigned long long* count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool hit = i < n && depth[i] >= 0.f;
+  const uint64_t m = __ballot(hit);
+  if (!m) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (hit) {
+    const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    const unsigned long long k = base + __popcll(below);
+    ids[k] = (uint32_t)i;
+    pos[3 * k + 0] = hit_pos[3 * i + 0];
+    pos[3 * k + 1] = hit_pos[3 * i + 1];
+    pos[3 * k + 2] = hit_pos[3 * i + 2];
+  }
+}
+
+__device__ inline uint32_t rgba8(float r, float g, float b) {
+  auto q = [](float v) { return (uint32_t)(fminf(fmaxf(v, 0.f), 1.f) * 255.f + 0.5f); };
+  return q(r) | (q(g) << 8) | (q(b) << 16) | (255u << 24);
+}
+
+// Background, then Lambert shading of the hits with the light at the eye
+// (Shade_kernel's diffuse term, renderer/cuda/kernels.cu:122-275:
+// ambient 0.2, diffuse 0.54, light = -ray direction).
+__global__ void k_background(uint32_t* rgba, float* normals, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rgba[i] = rgba8(1.f, 1.f, 1.f);
+  if (normals) {
+    normals[3 * i] = 0.f; normals[3 * i + 1] = 0.f; normals[3 * i + 2] = 0.f;
+  }
+}
+
+__global__ void k_shade(TraceArgs a, const uint32_t* ids, const float* grad, int64_t nh) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nh) return;
+  const int64_t id = ids[k];
+  float o[3], d[3], t0;
+  bool inside;
+  ray_init(a, id, o, d, t0, inside);
+  float nx = grad[3 * k], ny = grad[3 * k + 1], nz = grad[3 * k + 2];
+  const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+  const float inv = len > 0.f ? 1.0f / len : 0.f;
+  nx *= inv; ny *= inv; nz *= inv;
+  const float lambert = fmaxf(-(nx * d[0] + ny * d[1] + nz * d[2]), 0.f);
+  const float c = 0.2f + 0.54f * lambert;
+  a.rgba[id] = rgba8(c, c, c);
+  if (a.normals) {
+    a.normals[3 * id] = nx; a.normals[3 * id + 1] = ny; a.normals[3 * id + 2] = nz;
+  }
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+extern "C" {
+
+int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params* prm, float* depth,
+             float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats, void* stream) {
+  MP_REQUIRE(st != nullptr && cams != nullptr && prm != nullptr, MP_EINVAL, "mp_trace: NULL argument");
+  MP_REQUIRE(n_cams >= 1 && prm->width >= 1 && prm->height >= 1, MP_EINVAL, "mp_trace: empty image");
+  MP_REQUIRE(depth != nullptr && steps != nullptr && rgba != nullptr, MP_EINVAL,
+             "mp_trace: depth, steps and rgba are required");
+  MP_REQUIRE(prm->n_deltas == st->n_levels - 1, MP_EINVAL, "mp_trace: need %d deltas, got %d",
+             st->n_levels - 1, prm->n_deltas);
+  MP_REQUIRE(prm->max_steps >= 1 && prm->hit_eps > 0.f && prm->bound_radius > 0.f, MP_EINVAL,
+             "mp_trace: bad parameters");
+  hipStream_t s = as_stream(stream);
+  const int64_t total = (int64_t)n_cams * prm->width * prm->height;
+  MP_REQUIRE(total < (1ll << 32), MP_EUNSUPPORTED, "mp_trace: too many rays");
+  int maxw = 0;
+  for (int l = 0; l < st->n_levels; ++l) maxw = st->lv[l].width > maxw ? st->lv[l].width : maxw;
+
+  // scratch: counters | hit positions [total][3] | hit ids [total] | compact pos [total][3] | grad [total][3]
+  const size_t off_pos = 256;
+  const size_t off_ids = off_pos + (size_t)total * 12;
+  const size_t off_cpos = (off_ids + (size_t)total * 4 + 255) & ~(size_t)255;
+  const size_t off_grad = off_cpos + (size_t)total * 12;
+  int rc = ensure_scratch(st, off_grad + (size_t)total * 12);
+  if (rc != MP_OK) return rc;
+  char* base = reinterpret_cast<char*>(st->scratch);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base);
+  float* hit_pos = reinterpret_cast<float*>(base + off_pos);
+  uint32_t* ids = reinterpret_cast<uint32_t*>(base + off_ids);
+  float* cpos = reinterpret_cast<float*>(base + off_cpos);
+  float* grad = reinterpret_cast<float*>(base + off_grad);
+
+  TraceArgs a{};
+  a.cams = cams;
+  a.width = prm->width;
+  a.height = prm->height;
+  a.total_rays = total;
+  a.max_steps = prm->max_steps;
+  a.hit_eps = prm->hit_eps;
+  a.bound_radius = prm->bound_radius;
+  a.band_scale = prm->band_scale;
+  a.pixel_center = prm->pixel_center;
+  a.flip_y = prm->flip_y;
+  a.n_levels = st->n_levels;
+  for (int l = 0; l < st->n_levels; ++l) a.levels[l] = st->lv[l].dev;
+  for (int i = 0; i < prm->n_deltas; ++i) a.deltas[i] = prm->deltas[i];
+  a.next_ray = &ctr[0];
+  a.stats = &ctr[1];
+  a.depth = depth;
+  a.steps = steps;
+  a.hit_pos = hit_pos;
+  a.rgba = rgba;
+  a.normals = normals;
+
+  hipLaunchKernelGGL(k_fill_ull, dim3(1), dim3(32), 0, s, ctr, 32, 0ull);
+  MP_LAUNCHED("k_fill_ull");
+  int dev = 0, cus = 0;
+  MP_HIP(hipGetDevice(&dev));
+  MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int nc = (st->nc == 2 && maxw <= 256) ? 2 : 1;
+  // persistent grid: enough waves to fill every SIMD twice over
+  const int blocks = cus * 2;
+  switch (st->shape) {
+    case SH_32x32x16_BF16:
+      if (nc == 2) hipLaunchKernelGGL((k_trace<S32x32x16_bf16, 2>), dim3(blocks), dim3(256), 0, s, a);
+      else hipLaunchKernelGGL((k_trace<S32x32x16_bf16, 1>), dim3(blocks), dim3(256), 0, s, a);
+      break;
+    case SH_32x32x16_F16:
+      if (nc == 2) hipLaunchKernelGGL((k_trace<S32x32x16_f16, 2>), dim3(blocks), dim3(256), 0, s, a);
+      else hipLaunchKernelGGL((k_trace<S32x32x16_f16, 1>), dim3(blocks), dim3(256), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_trace<S16x16x4_f32, 1>), dim3(blocks), dim3(256), 0, s, a);
+      break;
+  }
+  MP_LAUNCHED("k_trace");
+  hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, depth,
+                     hit_pos, total, ids, cpos, &ctr[8]);
+  MP_LAUNCHED("k_collect_hits");
+  hipLaunchKernelGGL(k_background, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, rgba,
+                     normals, total);
+  MP_LAUNCHED("k_background");
+  unsigned long long h[9] = {0};
+  MP_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+  MP_HIP(hipStreamSynchronize(s));
+  const int64_t nh = (int64_t)h[8];
+  if (nh > 0) {
+    for (int l = 0; l < st->n_levels; ++l) {
+      rc = launch_level_grad(st, l, cpos, nh, nullptr, grad, l > 0, s);
+      if (rc != MP_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_shade, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, a, ids, grad, nh);
+    MP_LAUNCHED("k_shade");
+  }
+  if (stats) {
+    for (int l = 0; l < st->n_levels; ++l) stats[l] = (int64_t)h[1 + l];
+    stats[MP_MAX_LEVELS] = nh;
+  }
+  return MP_OK;
+}
+
+}  // extern "C"

@@ -43,6 +43,21 @@ class mp_level_desc(ctypes.Structure):
     ]
 
 
+class mp_trace_params(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("max_steps", ctypes.c_int),
+        ("hit_eps", ctypes.c_float),
+        ("bound_radius", ctypes.c_float),
+        ("band_scale", ctypes.c_float),
+        ("pixel_center", ctypes.c_int),
+        ("flip_y", ctypes.c_int),
+        ("n_deltas", ctypes.c_int),
+        ("deltas", ctypes.c_float * MP_MAX_LEVELS),
+    ]
+
+
 # Exported symbols: name -> (restype, argtypes).  The CPU tests check that the
 # library exports every one of them.
 SIGNATURES = {
@@ -61,6 +76,9 @@ SIGNATURES = {
                                          ctypes.c_void_p]),
     "mp_grid_coords": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                c_int64_p, ctypes.c_void_p]),
     "mp_mc_workspace_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mp_mc_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_float, c_int64_p, c_int64_p, ctypes.c_void_p,

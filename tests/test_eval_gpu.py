@@ -86,3 +86,33 @@ def test_grid_coords_bit_exact(N, mode):
     first = max(0, total - 70000) if N == 512 else 0
     xyz = grid_coords(N, mode, first=first, n=total - first).cpu().numpy()
     np.testing.assert_array_equal(xyz, ogrid.coords(N, np.arange(first, total), mode))
+
+
+GRAD_TOL = {"fp32": 2e-5, "fp16": 3e-3, "bf16": 2e-2}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("case", [([64, 64], 30.0, 1.0), ([256, 256, 256], 30.0, 1.0),
+                                  ([128, 100, 128, 128], 30.0, 1.0), ([512, 512, 512], 90.0, 0.1),
+                                  ([256], 30.0, 1.0)],
+                         ids=lambda c: "x".join(map(str, c[0])))
+def test_gradient_matches_fp64_chain_rule(case, precision):
+    hidden, omega, last = case
+    m = seeded_siren(hidden, omega, seed=77 + hidden[0], last_scale=last).cuda()
+    x64 = _points(4099, 5)
+    layers = osiren.layers_from_model(m)
+    f_ref, g_ref = osiren.forward_grad(layers, x64.numpy())
+    f, g = LevelStack([m], precision).eval(x64.float().cuda(), grad=True)
+    g = g.double().cpu().numpy()
+    gs = np.abs(g_ref).max()
+    err = np.abs(g - g_ref).max()
+    print(f"{precision} {hidden}: grad max err {err:.3e} (max |grad| {gs:.3e})")
+    assert err <= GRAD_TOL[precision] * gs
+
+
+def test_gradient_of_level_sum():
+    levels = [seeded_siren([128, 128, 128], 30.0, seed=s).cuda() for s in (4, 5)]
+    x64 = _points(1000, 9)
+    ref = sum(osiren.forward_grad(osiren.layers_from_model(m), x64.numpy())[1] for m in levels)
+    _, g = LevelStack(levels, "fp32").eval(x64.float().cuda(), grad=True)
+    assert np.abs(g.double().cpu().numpy() - ref).max() <= 2e-5 * np.abs(ref).max()

@@ -1,0 +1,82 @@
+"""Persistent multiscale sphere tracer (mp_trace) against a closed form and the
+FP64 oracle's restatement of the renderer's step rule."""
+import numpy as np
+import pytest
+import torch
+
+import os.path as osp
+
+from conftest import ROOT
+from mplicits.model import SIREN, from_pth
+from mplicits.stack import LevelStack
+from mplicits.synthetic import orbit_cameras, seeded_siren
+from mplicits.tracer import trace
+from oracle import siren as osiren
+from oracle import trace as otrace
+
+pytestmark = pytest.mark.gpu
+
+
+def _plane_model(c=0.2, eps=0.1):
+    # f(x) = sin(eps * z) / eps - c: zero set is the plane z = asin(eps c) / eps,
+    # gradient (0, 0, cos(eps z)); baked weights, 3 -> 64 -> 1.
+    m = SIREN(3, 1, [64], w0=1, ww=1, delay_init=True)
+    with torch.no_grad():
+        for p in m.parameters():
+            p.zero_()
+        m.net[0][0].weight[0, 2] = eps
+        m.net[1][0].weight[0, 0] = 1.0 / eps
+        m.net[1][0].bias[0] = -c
+    return m.cuda(), np.arcsin(eps * c) / eps
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_plane_depth_and_normals_closed_form(precision):
+    m, zp = _plane_model()
+    cams = orbit_cameras(3, seed=5, radius=2.5, elev_deg=60)
+    W, H = 96, 64
+    out = trace(LevelStack([m], precision), cams, W, H, deltas=[], max_steps=256, hit_eps=1e-5)
+    depth = out["depth"].cpu().numpy().reshape(-1)
+    P, D, t0, inside = otrace.rays(cams, W, H)
+    # eye positions and the analytic intersection with z = zp inside the sphere
+    eye = np.concatenate([np.repeat(np.asarray(iv)[:3, 3][None], W * H, 0) for iv, _ in cams])
+    t_plane = (zp - eye[:, 2]) / D[:, 2]
+    xp = eye + t_plane[:, None] * D
+    # a ray that enters the sphere on the positive side (f > 0) walks forward
+    # to the plane; one that enters on the negative side steps backward out of
+    # the sphere (signed steps, as in the reference tracers) and misses
+    expect_hit = (P[:, 2] > zp) & (t_plane > t0) & ((xp ** 2).sum(1) < 1.2 ** 2 - 1e-3) & inside
+    got_hit = depth >= 0
+    # away from the sphere rim every ray must agree
+    rim = np.abs(np.sqrt((xp ** 2).sum(1)) - 1.2) < 0.02
+    assert np.array_equal(got_hit[~rim], expect_hit[~rim])
+    both = got_hit & expect_hit
+    assert both.sum() > 1000
+    assert np.abs(depth[both] - t_plane[both]).max() < 5e-4
+    nrm = out["normals"].cpu().numpy().reshape(-1, 3)[both]
+    assert np.abs(np.abs(nrm[:, 2]) - 1).max() < 1e-3
+    assert out["hits"] == int(got_hit.sum())
+
+
+def test_stack_matches_oracle_step_rule():
+    coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
+    levels = [coarse, seeded_siren([128] * 3, 60.0, seed=42, last_scale=0.1).cuda()]
+    deltas = [0.05]
+    cams = orbit_cameras(2, seed=9)
+    W, H = 64, 48
+    out = trace(LevelStack(levels, "fp32"), cams, W, H, deltas=deltas)
+    depth = out["depth"].cpu().numpy().reshape(-1)
+    rng = np.random.default_rng(0)
+    ids = rng.choice(len(depth), 600, replace=False)
+    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
+    layers = [osiren.layers_from_model(m) for m in levels]
+    hit, t, _ = otrace.trace(layers, deltas, P[inside], D[inside])
+    ref_depth = np.full(len(ids), -1.0)
+    ref_depth[np.nonzero(inside)[0][hit]] = (t0[inside] + t)[hit]
+    got = depth[ids]
+    agree = (got >= 0) == (ref_depth >= 0)
+    assert agree.mean() > 0.97, agree.mean()
+    both = (got >= 0) & (ref_depth >= 0)
+    assert both.sum() > 50
+    assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
+    assert out["evals"][0] > 0 and out["evals"][1] > 0
