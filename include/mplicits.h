@@ -147,6 +147,28 @@ int mp_trace(mp_stack* stack, const float* cams, int n_cams, const mp_trace_para
              float* depth, float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats,
              void* stream);
 
+/* Elementwise stages of the residual-level training step (value + T input
+ * tangents per point, T <= 3), replacing the autograd graph of
+ * loss_add_detail_fine (i3d/loss_functions.py:327-394) with its
+ * create_graph gradient (i3d/diff_operators.py:259-265) and double backward
+ * (experiment_scripts/train_sdf_on_neighborhood_fine.py:220-252).  Activation
+ * blocks are [n][1+T][W]; Z (pre-activations) is float32, X / Abar (GEMM
+ * operands) have element type x_dtype / a_dtype: 0 float32, 1 bfloat16,
+ * 2 float16.  The GEMMs between the stages are issued by the caller.
+ *   mp_train_first:   Z = [W0 x + b0; W0 v_t], X = [sin; cos * tangent]   (layer 0)
+ *   mp_train_act:     Z[:, 0] += b, X = [sin; cos * tangent]              (hidden layer)
+ *   mp_train_act_bwd: Abar = [cos pbar - sin sum_t ad_t pdbar_t; cos pdbar_t],
+ *                     abar_val [n][W] (float32, may be NULL) = value row
+ *   mp_train_seed:    Pbar = [s w_L; g_t w_L] (s or g may be NULL = 0) */
+int mp_train_first(const float* x, const float* v, const float* W0, const float* b0, int64_t n, int W,
+                   int n_tangents, float* Z, void* X, int x_dtype, void* stream);
+int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype,
+                 void* stream);
+int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
+                     int a_dtype, float* abar_val, void* stream);
+int mp_train_seed(const float* s, const float* g, const float* w_last, int64_t n, int W, int n_tangents,
+                  float* Pbar, void* stream);
+
 size_t mp_mc_workspace_size(int nz, int ny, int nx);
 int mp_mc_count(const float* vol, int nz, int ny, int nx, float level, int64_t* n_verts,
                 int64_t* n_faces, void* ws, void* stream);

@@ -1,0 +1,183 @@
+// Elementwise stages of the residual-level training step (forward with input
+// tangents and its hand-written backward, including the second-order terms of
+// the input-gradient losses).  Replaces the autograd graph of
+// loss_add_detail_fine (i3d/loss_functions.py:327-394: a forward, a
+// create_graph gradient and a double backward) in the trainers
+// (experiment_scripts/train_sdf_on_neighborhood_fine.py:220-252).
+//
+// Notation per sine layer j (frequency baked into W, b):
+//   a_j = W_j p_{j-1} + b_j,     p_j = sin a_j,
+//   ad_j,t = W_j pd_{j-1},t,     pd_j,t = cos(a_j) ad_j,t       (tangent t = 1..T)
+// with p_{-1} = x and pd_{-1},t = v_t (the tangent directions).  Every
+// activation block is laid out [n][1 + T][W]: row 0 the value, rows 1..T the
+// tangents, so one GEMM over n (1 + T) rows applies W_j to all of them.
+// Backward, given pbar_j (value row) and pdbar_j,t (tangent rows):
+//   abar_j    = cos(a_j) pbar_j - sin(a_j) sum_t ad_j,t pdbar_j,t
+//   adbar_j,t = cos(a_j) pdbar_j,t
+// then dW_j = [abar; adbar]^T [p_{j-1}; pd_{j-1}], db_j = sum abar_j and
+// [pbar; pdbar]_{j-1} = [abar; adbar] W_j (GEMMs, issued by the host).
+#include "common.h"
+
+namespace mp {
+
+__device__ inline void st(float* p, int64_t i, float v) { p[i] = v; }
+__device__ inline void st(__bf16* p, int64_t i, float v) { p[i] = (__bf16)v; }
+__device__ inline void st(_Float16* p, int64_t i, float v) { p[i] = (_Float16)v; }
+
+// Layer 0: Z = [a0; ad0] (float32), X = [sin a0; cos a0 * ad0].
+template <class T>
+__global__ void k_train_first(const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ W0,
+                              const float* __restrict__ b0, int64_t n, int W, int nt, float* __restrict__ Z,
+                              T* __restrict__ X) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * W) return;
+  const int64_t p = i / W;
+  const int u = (int)(i % W);
+  const float w0 = W0[3 * u], w1 = W0[3 * u + 1], w2 = W0[3 * u + 2];
+  const float a = fmaf(w2, x[3 * p + 2], fmaf(w1, x[3 * p + 1], fmaf(w0, x[3 * p], b0[u])));
+  float s, c;
+  sincosf(a, &s, &c);
+  const int64_t base = p * (1 + nt) * W + u;
+  Z[base] = a;
+  st(X, base, s);
+  for (int t = 0; t < nt; ++t) {
+    const float* vt = v + (p * nt + t) * 3;
+    const float ad = fmaf(w2, vt[2], fmaf(w1, vt[1], w0 * vt[0]));
+    Z[base + (int64_t)(1 + t) * W] = ad;
+    st(X, base + (int64_t)(1 + t) * W, c * ad);
+  }
+}
+
+// Hidden layer: Z holds W_j X_{j-1} (no bias); adds the bias to the value
+// row in place and writes X_j.
+template <class T>
+__global__ void k_train_act(float* __restrict__ Z, const float* __restrict__ b, int64_t n, int W, int nt,
+                            T* __restrict__ X) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * W) return;
+  const int64_t p = i / W;
+  const int u = (int)(i % W);
+  const int64_t base = p * (1 + nt) * W + u;
+  const float a = Z[base] + b[u];
+  Z[base] = a;
+  float s, c;
+  sincosf(a, &s, &c);
+  st(X, base, s);
+  for (int t = 0; t < nt; ++t) {
+    const int64_t k = base + (int64_t)(1 + t) * W;
+    st(X, k, c * Z[k]);
+  }
+}
+
+// Backward through sin / the tangent products: Pbar -> Abar.
+template <class T>
+__global__ void k_train_act_bwd(const float* __restrict__ Z, const float* __restrict__ Pbar, int64_t n, int W,
+                                int nt, T* __restrict__ Abar, float* __restrict__ abar_val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * W) return;
+  const int64_t p = i / W;
+  const int u = (int)(i % W);
+  const int64_t base = p * (1 + nt) * W + u;
+  float s, c;
+  sincosf(Z[base], &s, &c);
+  float acc = c * Pbar[base];
+  for (int t = 0; t < nt; ++t) {
+    const int64_t k = base + (int64_t)(1 + t) * W;
+    const float pdb = Pbar[k];
+    acc = fmaf(-s * Z[k], pdb, acc);
+    st(Abar, k, c * pdb);
+  }
+  st(Abar, base, acc);
+  if (abar_val) abar_val[p * W + u] = acc;
+}
+
+// Output layer seeds: pbar = s * w_L (value row), pdbar_t = g_t * w_L.
+__global__ void k_train_seed(const float* __restrict__ s, const float* __restrict__ g, const float* __restrict__ wl,
+                             int64_t n, int W, int nt, float* __restrict__ Pbar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * W) return;
+  const int64_t p = i / W;
+  const int u = (int)(i % W);
+  const int64_t base = p * (1 + nt) * W + u;
+  const float w = wl[u];
+  Pbar[base] = s ? s[p] * w : 0.f;
+  for (int t = 0; t < nt; ++t) Pbar[base + (int64_t)(1 + t) * W] = g ? g[p * nt + t] * w : 0.f;
+}
+
+static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace mp
+
+using namespace mp;
+
+extern "C" {
+
+int mp_train_first(const float* x, const float* v, const float* W0, const float* b0, int64_t n, int W,
+                   int n_tangents, float* Z, void* X, int x_dtype, void* stream) {
+  MP_REQUIRE(x && W0 && b0 && Z && X && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_first: bad arguments");
+  MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first: tangent directions are NULL");
+  if (n == 0) return MP_OK;
+  hipStream_t s = as_stream(stream);
+  if (x_dtype == 1)
+    hipLaunchKernelGGL(k_train_first<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
+                       n_tangents, Z, reinterpret_cast<__bf16*>(X));
+  else if (x_dtype == 2)
+    hipLaunchKernelGGL(k_train_first<_Float16>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
+                       n_tangents, Z, reinterpret_cast<_Float16*>(X));
+  else
+    hipLaunchKernelGGL(k_train_first<float>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
+                       n_tangents, Z, reinterpret_cast<float*>(X));
+  MP_LAUNCHED("k_train_first");
+  return MP_OK;
+}
+
+int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype, void* stream) {
+  MP_REQUIRE(Z && b && X && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_act: bad arguments");
+  if (n == 0) return MP_OK;
+  hipStream_t s = as_stream(stream);
+  if (x_dtype == 1)
+    hipLaunchKernelGGL(k_train_act<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
+                       reinterpret_cast<__bf16*>(X));
+  else if (x_dtype == 2)
+    hipLaunchKernelGGL(k_train_act<_Float16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
+                       reinterpret_cast<_Float16*>(X));
+  else
+    hipLaunchKernelGGL(k_train_act<float>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
+                       reinterpret_cast<float*>(X));
+  MP_LAUNCHED("k_train_act");
+  return MP_OK;
+}
+
+int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
+                     int a_dtype, float* abar_val, void* stream) {
+  MP_REQUIRE(Z && Pbar && Abar && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_act_bwd: bad arguments");
+  if (n == 0) return MP_OK;
+  hipStream_t s = as_stream(stream);
+  if (a_dtype == 1)
+    hipLaunchKernelGGL(k_train_act_bwd<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W, n_tangents,
+                       reinterpret_cast<__bf16*>(Abar), abar_val);
+  else if (a_dtype == 2)
+    hipLaunchKernelGGL(k_train_act_bwd<_Float16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W,
+                       n_tangents, reinterpret_cast<_Float16*>(Abar), abar_val);
+  else
+    hipLaunchKernelGGL(k_train_act_bwd<float>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W, n_tangents,
+                       reinterpret_cast<float*>(Abar), abar_val);
+  MP_LAUNCHED("k_train_act_bwd");
+  return MP_OK;
+}
+
+int mp_train_seed(const float* s_val, const float* g_tan, const float* wl, int64_t n, int W, int n_tangents,
+                  float* Pbar, void* stream) {
+  MP_REQUIRE(wl && Pbar && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_seed: bad arguments");
+  if (n == 0) return MP_OK;
+  hipLaunchKernelGGL(k_train_seed, dim3(grid_for(n * W)), dim3(256), 0, as_stream(stream), s_val, g_tan, wl, n, W,
+                     n_tangents, Pbar);
+  MP_LAUNCHED("k_train_seed");
+  return MP_OK;
+}
+
+}  // extern "C"
