@@ -33,7 +33,9 @@ sys.path.insert(0, osp.join(ROOT, "m-plicits_amd"))
 from mplicits.meshing import marching_cubes_gpu  # noqa: E402
 from mplicits.model import from_pth  # noqa: E402
 from mplicits.stack import LevelStack  # noqa: E402
-from mplicits.synthetic import DELTAS, residual_levels  # noqa: E402
+from mplicits.synthetic import DELTAS, MASTER_SEED, BumpySphere, orbit_cameras, residual_levels  # noqa: E402
+from mplicits.tracer import camera_block, trace  # noqa: E402
+from mplicits import training  # noqa: E402
 
 N = 512
 CHUNK = 32 ** 3
@@ -146,12 +148,64 @@ def cpu_baseline(levels, budget_chunks=96):
     return t_eval, ids, threads
 
 
+def bench_trace(levels, precisions, reps=2):
+    """16 seeded orbit cameras at 1920x1080 through the coarse model (one
+    persistent launch + normals + shading per frame set): rays per second."""
+    cams = camera_block(orbit_cameras(16, seed=MASTER_SEED + 7))
+    out = {}
+    for prec in precisions:
+        st = LevelStack([levels[0]], prec)
+        r = trace(st, cams, 1920, 1080, [])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = trace(st, cams, 1920, 1080, [])
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[prec] = {"rays_per_s": round(16 * 1920 * 1080 / dt, 1), "ms": round(dt * 1e3, 2), "hits": r["hits"],
+                     "coarse_evals": r["evals"][0]}
+    return out
+
+
+def bench_train(levels, precisions, steps=4):
+    """One Adam step of the fine 512x3 residual over the frozen coarse + medium
+    levels on 2^18 points (uniform in [-1,1]^3), config loss
+    (L1 SDF + eikonal + normal cosine): training points per second."""
+    coarse, med, fine = levels
+    shape = BumpySphere()
+    n = 2 ** 18
+    g = torch.Generator(device="cuda").manual_seed(MASTER_SEED)
+    x = torch.rand(n, 3, device="cuda", generator=g) * 2 - 1
+    f_t, n_t = shape.sdf(x).float(), shape.normals(x).float()
+    out = {}
+    for prec in precisions:
+        frozen = LevelStack([coarse, med], prec)
+        model = type(fine)(3, 1, fine.hidden_widths(), w0=1, ww=1, delay_init=True).cuda()
+        model.load_state_dict(fine.state_dict())
+        opt = torch.optim.Adam(model.parameters(), lr=1e-4)
+        loss_fn = lambda F, G: training.config_loss(F, G, f_t, n_t)  # noqa: E731
+        for _ in range(2):
+            opt.zero_grad(set_to_none=True)
+            training.train_step(model, x, loss_fn, frozen=frozen, optimizer=opt, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            opt.zero_grad(set_to_none=True)
+            loss, _, _ = training.train_step(model, x, loss_fn, frozen=frozen, optimizer=opt, precision=prec)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[prec] = {"train_pts_per_s": round(n / dt, 1), "ms_per_step": round(dt * 1e3, 2),
+                     "loss": round(float(loss), 5)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the tracing and training figures")
     ap.add_argument("--precisions", default="bf16,fp16,fp32",
                     help="comma list of paths to time (profiling runs use one)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -184,6 +238,12 @@ def main():
         print(f"# {prec}: {ms:.2f} ms/extraction, counts {counts}, mesh {nv} v / {nf} f, err {err}",
               file=sys.stderr, flush=True)
         del st
+
+    tr = bench_trace(levels, ["bf16", "fp32"]) if not args.no_extras else None
+    tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
+    if tr:
+        print(f"# trace: {tr}", file=sys.stderr, flush=True)
+        print(f"# train: {tn}", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel (coarse level over the whole grid)
     coarse_flop = level_flops(levels[0].hidden_widths()) * N ** 3
@@ -218,6 +278,8 @@ def main():
         "speedup_bf16_vs_fp32": round(fp32["ms"] / b["ms"], 2) if fp32 else None,
         "level_counts": b["counts"], "mesh": {"verts": b["verts"], "faces": b["faces"]},
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
+        "trace_16x1080p": tr,
+        "train_step_2e18": tn,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -105,19 +105,6 @@ int mp_extract_volume(mp_stack* stack, int N, int grid_mode, int64_t chunk, cons
  * (gen_mc_coordinate_grid, i3d/meshing.py:11-69, columns 0..2). */
 int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, void* stream);
 
-/* Marching cubes of a [nz][ny][nx] float32 device volume at `level`: replaces
- * skimage.measure.marching_cubes(volume, level) as convert_sdf_samples_to_ply
- * calls it (i3d/meshing.py:296-298; Lewiner method, 'descent' winding,
- * degenerate triangles kept).  Two calls: mp_mc_count (host-sync point) returns
- * the vertex and face counts, the caller allocates, mp_mc_emit writes
- *   verts   [V][3] float32, voxel units, (axis0, axis1, axis2) order,
- *   faces   [F][3] int32 vertex ids,
- *   normals [V][3] float32 unit -grad(volume) (may be NULL),
- *   values  [V] float32 local value range (may be NULL).
- * Vertex and face ids are deterministic (prefix sums in linear grid order).
- * verts and faces may be NULL only when the counts are zero.
- * `ws` is a device workspace of mp_mc_workspace_size(nz, ny, nx) bytes, kept
- * between the two calls. */
 /* Multiscale sphere tracing of the whole stack.  Replaces the renderer's
  * MIPplicit::SdfInference tracing loop and shading (renderer/cuda/state.cu:816-1055;
  * InitializeRays_kernel, IterateSphereTracing_kernel and Shade_kernel in
@@ -168,7 +155,29 @@ int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_
                      int a_dtype, float* abar_val, void* stream);
 int mp_train_seed(const float* s, const float* g, const float* w_last, int64_t n, int W, int n_tangents,
                   float* Pbar, void* stream);
+/* Output layer (W -> 1) from the last sine layer's float32 pre-activations Z:
+ * forward f [n] = w_L . sin a + b_L, fd [n][T] = w_L . (cos a ad_t);
+ * backward Pbar = [s w_L; g_t w_L] and per-block partial sums dwl_part
+ * [mp_train_out_bwd_blocks(n)][W] of dw_L = sum s sin a + sum_t g_t cos a ad_t. */
+int mp_train_out_fwd(const float* Z, const float* w_last, const float* b_last, int64_t n, int W,
+                     int n_tangents, float* f, float* fd, void* stream);
+int64_t mp_train_out_bwd_blocks(int64_t n);
+int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
+                     int n_tangents, float* Pbar, float* dwl_part, void* stream);
 
+/* Marching cubes of a [nz][ny][nx] float32 device volume at `level`: replaces
+ * skimage.measure.marching_cubes(volume, level) as convert_sdf_samples_to_ply
+ * calls it (i3d/meshing.py:296-298; Lewiner method, 'descent' winding,
+ * degenerate triangles kept).  Two calls: mp_mc_count (host-sync point) returns
+ * the vertex and face counts, the caller allocates, mp_mc_emit writes
+ *   verts   [V][3] float32, voxel units, (axis0, axis1, axis2) order,
+ *   faces   [F][3] int32 vertex ids,
+ *   normals [V][3] float32 unit -grad(volume) (may be NULL),
+ *   values  [V] float32 local value range (may be NULL).
+ * Vertex and face ids are deterministic (prefix sums in linear grid order).
+ * verts and faces may be NULL only when the counts are zero.
+ * `ws` is a device workspace of mp_mc_workspace_size(nz, ny, nx) bytes, kept
+ * between the two calls. */
 size_t mp_mc_workspace_size(int nz, int ny, int nx);
 int mp_mc_count(const float* vol, int nz, int ny, int nx, float level, int64_t* n_verts,
                 int64_t* n_faces, void* ws, void* stream);

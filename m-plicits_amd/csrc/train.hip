@@ -181,3 +181,98 @@ int mp_train_seed(const float* s_val, const float* g_tan, const float* wl, int64
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Output layer (W -> 1) of the training forward / backward, fused:
+//   forward:  f = w_L . sin(a) + b_L,  fd_t = w_L . (cos(a) ad_t)
+//   backward: Pbar = [s w_L; g_t w_L] and the partial sums of
+//             dw_L = sum_points s sin(a) + sum_t g_t cos(a) ad_t
+// One wave per point for the forward (lane-strided dot products, then a
+// wave reduction); the backward accumulates dw_L per block.
+// ---------------------------------------------------------------------------
+namespace mp {
+
+__global__ __launch_bounds__(256) void k_train_out_fwd(const float* __restrict__ Z, const float* __restrict__ wl,
+                                                       const float* __restrict__ bl, int64_t n, int W, int nt,
+                                                       float* __restrict__ f, float* __restrict__ fd) {
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= n) return;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  const float* z = Z + p * (1 + nt) * W;
+  for (int u = lane; u < W; u += 64) {
+    float s, c;
+    sincosf(z[u], &s, &c);
+    const float w = wl[u];
+    acc[0] = fmaf(w, s, acc[0]);
+    for (int t = 0; t < nt; ++t) acc[1 + t] = fmaf(w * c, z[(1 + t) * W + u], acc[1 + t]);
+  }
+  for (int k = 0; k <= nt; ++k)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+  if (lane == 0) {
+    f[p] = acc[0] + bl[0];
+    for (int t = 0; t < nt; ++t) fd[p * nt + t] = acc[1 + t];
+  }
+}
+
+// grid.x = ceil(n / rows_per_block), block = 256 threads over W (W <= 512:
+// two units per thread); dwl_part [gridDim.x][W].
+__global__ __launch_bounds__(256) void k_train_out_bwd(const float* __restrict__ Z, const float* __restrict__ s,
+                                                       const float* __restrict__ g, const float* __restrict__ wl,
+                                                       int64_t n, int W, int nt, int rows_per_block,
+                                                       float* __restrict__ Pbar, float* __restrict__ dwl_part) {
+  const int64_t p0 = (int64_t)blockIdx.x * rows_per_block;
+  for (int u = threadIdx.x; u < W; u += blockDim.x) {
+    const float w = wl[u];
+    float acc = 0.f;
+    for (int r = 0; r < rows_per_block; ++r) {
+      const int64_t p = p0 + r;
+      if (p >= n) break;
+      const float* z = Z + p * (1 + nt) * W;
+      float* pb = Pbar + p * (1 + nt) * W;
+      float sn, cs;
+      sincosf(z[u], &sn, &cs);
+      const float sv = s ? s[p] : 0.f;
+      pb[u] = sv * w;
+      acc = fmaf(sv, sn, acc);
+      for (int t = 0; t < nt; ++t) {
+        const float gv = g ? g[p * nt + t] : 0.f;
+        pb[(1 + t) * W + u] = gv * w;
+        acc = fmaf(gv * cs, z[(1 + t) * W + u], acc);
+      }
+    }
+    dwl_part[(int64_t)blockIdx.x * W + u] = acc;
+  }
+}
+
+}  // namespace mp
+
+extern "C" {
+
+int mp_train_out_fwd(const float* Z, const float* w_last, const float* b_last, int64_t n, int W, int n_tangents,
+                     float* f, float* fd, void* stream) {
+  MP_REQUIRE(Z && w_last && b_last && f && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_out_fwd: bad arguments");
+  MP_REQUIRE(n_tangents == 0 || fd, MP_EINVAL, "mp_train_out_fwd: fd is NULL");
+  if (n == 0) return MP_OK;
+  hipLaunchKernelGGL(k_train_out_fwd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, as_stream(stream), Z, w_last,
+                     b_last, n, W, n_tangents, f, fd);
+  MP_LAUNCHED("k_train_out_fwd");
+  return MP_OK;
+}
+
+int64_t mp_train_out_bwd_blocks(int64_t n) { return (n + 255) / 256; }
+
+int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
+                     int n_tangents, float* Pbar, float* dwl_part, void* stream) {
+  MP_REQUIRE(Z && w_last && Pbar && dwl_part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_out_bwd: bad arguments");
+  if (n == 0) return MP_OK;
+  const int64_t blocks = mp_train_out_bwd_blocks(n);
+  hipLaunchKernelGGL(k_train_out_bwd, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), Z, s, g, w_last, n,
+                     W, n_tangents, 256, Pbar, dwl_part);
+  MP_LAUNCHED("k_train_out_bwd");
+  return MP_OK;
+}
+
+}  // extern "C"

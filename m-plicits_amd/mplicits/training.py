@@ -89,11 +89,11 @@ def forward_tangents(layers, x, V, precision="fp32"):
         Xs.append(Xj)
     WL, bL, _ = layers[-1]
     # output layer in FP32 from the FP32 pre-activations of the last sine layer
-    a = Zs[-1][:, 0]
-    s, c = torch.sin(a), torch.cos(a)
-    wl = WL.detach()[0]
-    f = s @ wl + bL.detach()[0]
-    fd = (c[:, None, :] * Zs[-1][:, 1:]) @ wl if nt else torch.zeros(n, 0, device=x.device)
+    f = torch.empty(n, device=x.device, dtype=torch.float32)
+    fd = torch.empty(n, nt, device=x.device, dtype=torch.float32)
+    _lib.check(lib.mp_train_out_fwd(_lib.ptr(Zs[-1]), _lib.ptr(WL.detach().contiguous()),
+                                    _lib.ptr(bL.detach().contiguous()), n, Zs[-1].shape[-1], nt, _lib.ptr(f),
+                                    _lib.ptr(fd if nt else None), stream))
     return f, fd, (Zs, Xs)
 
 
@@ -110,18 +110,16 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
     WL, bL, _ = layers[-1]
     wl = WL.detach()[0].contiguous()
     zl = Zs[-1]
-    sin_l, cos_l = torch.sin(zl[:, 0]), torch.cos(zl[:, 0])
-    # output layer gradients
-    dwl = (sin_l * s[:, None]).sum(0) if s is not None else torch.zeros_like(wl)
-    if nt and g is not None:
-        dwl = dwl + (cos_l[:, None, :] * zl[:, 1:] * g[:, :, None]).sum((0, 1))
+    wlast = zl.shape[-1]
+    # output layer: seeds of the last sine layer and dw_L in one pass
+    Pbar = torch.empty(n, 1 + nt, wlast, device=x.device, dtype=torch.float32)
+    part = torch.empty(int(lib.mp_train_out_bwd_blocks(n)), wlast, device=x.device, dtype=torch.float32)
+    _lib.check(lib.mp_train_out_bwd(_lib.ptr(zl), _lib.ptr(s), _lib.ptr(g.contiguous() if g is not None else None),
+                                    _lib.ptr(wl), n, wlast, nt, _lib.ptr(Pbar), _lib.ptr(part), stream))
+    dwl = part.sum(0)
     dbl = s.sum().reshape(1) if s is not None else torch.zeros(1, device=x.device)
     grads = [None] * len(layers)
     grads[-1] = (dwl.reshape(1, -1), dbl)
-    wlast = zl.shape[-1]
-    Pbar = torch.empty(n, 1 + nt, wlast, device=x.device, dtype=torch.float32)
-    _lib.check(lib.mp_train_seed(_lib.ptr(s), _lib.ptr(g.contiguous() if g is not None else None), _lib.ptr(wl),
-                                 n, wlast, nt, _lib.ptr(Pbar), stream))
     for j in range(len(layers) - 2, 0, -1):
         Wj = layers[j][0].detach()
         wout, win = Wj.shape
@@ -257,7 +255,17 @@ def train_step(model, x, loss_fn, frozen=None, optimizer=None, precision="fp32")
     V = unit_tangents(n, x.device)
     f, fd, cache = forward_tangents(layers, x, V, precision)
     if frozen is not None:
-        fc, gc = frozen.eval(x, grad=True)
+        if precision == "fp32" and max(max(m.hidden_widths()) for m in frozen.models) >= 256:
+            # the register-resident FP32 gradient kernel spills at these widths;
+            # the layered pass (FP32 GEMMs) is faster there
+            fc = torch.zeros_like(f)
+            gc = torch.zeros_like(fd)
+            for m in frozen.models:
+                fm, gm, _ = forward_tangents(baked_layers(m), x, V, "fp32")
+                fc += fm
+                gc += gm
+        else:
+            fc, gc = frozen.eval(x, grad=True)
         f_tot, g_tot = f + fc, fd + gc
     else:
         f_tot, g_tot = f, fd
