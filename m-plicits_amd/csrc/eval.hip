@@ -98,11 +98,8 @@ int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t
   const int nc = level_nc(st, level);
   switch (st->shape) {
     case SH_32x32x16_BF16:
-      return nc == 2 ? launch_width<S32x32x16_bf16, 2>(L, t, stream)
-                         : launch_width<S32x32x16_bf16, 1>(L, t, stream);
     case SH_32x32x16_F16:
-      return nc == 2 ? launch_width<S32x32x16_f16, 2>(L, t, stream)
-                         : launch_width<S32x32x16_f16, 1>(L, t, stream);
+      return launch_level16(st, level, t, stream);
     case SH_16x16x4_F32:
       return launch_width<S16x16x4_f32, 1>(L, t, stream);
   }

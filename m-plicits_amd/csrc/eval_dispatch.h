@@ -27,6 +27,9 @@ struct EvalTarget {
 
 int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream);
 
+// launch_level for the 32x32x16 BF16 / FP16 shapes (k_level16).
+int launch_level16(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream);
+
 // f_level and its input gradient at n points xyz [n][3]; f may be NULL;
 // accumulate adds into f / g instead of overwriting.
 int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n, float* f, float* g,

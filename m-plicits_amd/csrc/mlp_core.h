@@ -173,6 +173,7 @@ struct LevelDev {
   int width;            // padded hidden width (64, 128, 256 or 512)
   int n_hidden;         // sine layers
   const float4* w0v;    // [width]  (w_x, w_y, w_z, b) of layer 0, virtual order
+  const float2* w0m;    // [width/32][64] layer 0 as A operands of two 32x32x2 FP32 MFMAs
   const void* apk;      // [n_hidden-1][width/ROWS][width/KB][64] A fragments
   const void* apkt;     // same for the transposed hidden weights (backward pass, radians)
   const float* biasc;   // [n_hidden-1][width/ROWS][GROUPS][CREGS] hidden biases, C order

@@ -37,6 +37,10 @@ inline int padded_width(int w) {
   return -1;
 }
 
+// Bytes of readable padding after the packed hidden-layer A fragments: the
+// level kernels prefetch a fixed number of fragments ahead of their use.
+constexpr size_t APK_PAD = 16 * 64 * 16;
+
 // Launch-shape selection shared by every kernel family (a tile-shape override
 // from the autotuner is applied at stack creation).
 int default_nc(int shape, int width);

@@ -70,6 +70,7 @@ struct PackArgs {
   int64_t woff[17];                  // offset of W_k in W
   int64_t boff[17];                  // offset of b_k in B
   float4* w0v;
+  float2* w0m;
   void* apk;
   void* apkt;
   float* biasc;
@@ -101,6 +102,19 @@ __global__ void k_pack_small(PackArgs a) {
     // hidden biases, C order
     for (int j = 1; j < H; ++j)
       a.biasc[(int64_t)(j - 1) * a.width + i] = (r < a.dims[j + 1]) ? a.B[a.boff[j] + r] * INV_2PI : 0.f;
+  }
+  // layer 0 as the A operands of two 32x32x2 FP32 MFMAs per 32-row block:
+  // lane l holds row r = 32 rb + l % 32 and reduction indices (g, 2 + g),
+  // g = l / 32, over the operand (x, y, z, 1): (w_x | w_y, w_z | b).
+  if (i < a.width * 2) {
+    const int rb = i / 64, lane = i % 64, r = rb * 32 + lane % 32, g = lane / 32;
+    float2 m = make_float2(0.f, 0.f);
+    if (r < a.dims[1]) {
+      const float* row = a.W + a.woff[0] + (int64_t)r * 3;
+      m = make_float2(row[g], g == 0 ? row[2] : a.B[a.boff[0] + r]);
+      if (first_in_revs(a.shape)) m = make_float2(m.x * INV_2PI, m.y * INV_2PI);
+    }
+    a.w0m[i] = m;
   }
   if (i == 0) *a.blp = a.B[a.boff[H]];
 }
@@ -152,13 +166,14 @@ static int pack_level(const mp_stack* st, const mp_stack::Level& L, hipStream_t 
     bo += a.dims[k + 1];
   }
   a.w0v = const_cast<float4*>(L.dev.w0v);
+  a.w0m = const_cast<float2*>(L.dev.w0m);
   a.apk = const_cast<void*>(L.dev.apk);
   a.apkt = const_cast<void*>(L.dev.apkt);
   a.biasc = const_cast<float*>(L.dev.biasc);
   a.wlc = const_cast<float*>(L.dev.wlc);
   a.wlv = const_cast<float*>(L.dev.wlv);
   a.blp = const_cast<float*>(L.dev.blp);
-  hipLaunchKernelGGL(k_pack_small, dim3((L.width + 255) / 256), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_pack_small, dim3((2 * L.width + 255) / 256), dim3(256), 0, stream, a);
   MP_LAUNCHED("k_pack_small");
   const int64_t per_layer = (int64_t)L.width * L.width;
   for (int j = 1; j < L.n_hidden; ++j) {
@@ -245,7 +260,8 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     const size_t sz_bias = n_gemm * W * sizeof(float);
     const size_t sz_wl = W * sizeof(float);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t total = al(sz_w0v) + 2 * al(sz_apk) + al(sz_bias) + 2 * al(sz_wl) + 256;
+    // the level kernels prefetch A fragments up to APK_PAD bytes past the last layer
+    const size_t total = 2 * al(sz_w0v) + al(sz_apk + APK_PAD) + al(sz_apk) + al(sz_bias) + 2 * al(sz_wl) + 256;
     hipError_t e = hipMalloc(&L.buf, total);
     if (e != hipSuccess) {
       mp::set_error("mp_stack_create: hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
@@ -257,7 +273,8 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     L.dev.width = L.width;
     L.dev.n_hidden = L.n_hidden;
     L.dev.w0v = reinterpret_cast<const float4*>(p); p += al(sz_w0v);
-    L.dev.apk = p; p += al(sz_apk);
+    L.dev.w0m = reinterpret_cast<const float2*>(p); p += al(sz_w0v);
+    L.dev.apk = p; p += al(sz_apk + APK_PAD);
     L.dev.apkt = p; p += al(sz_apk);
     L.dev.biasc = reinterpret_cast<const float*>(p); p += al(sz_bias);
     L.dev.wlc = reinterpret_cast<const float*>(p); p += al(sz_wl);
