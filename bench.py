@@ -249,7 +249,7 @@ def main():
     coarse_flop = level_flops(levels[0].hidden_widths()) * N ** 3
     k_ms = dominant_kernel_ms(levels[0], "bf16")
     achieved = coarse_flop / (k_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_level<S32x32x16_bf16,256,2> (coarse 256x3 over 512^3)",
+    roofline = {"bound": "mfma", "kernel": "k_level16<S32x32x16_bf16,256,2> (coarse 256x3 over 512^3)",
                 "achieved": round(achieved, 1), "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_TFLOPS["bf16"], 4), "traffic": None,
                 "kernel_ms": round(k_ms, 3), "flop_per_launch": coarse_flop}

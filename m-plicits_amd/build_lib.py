@@ -25,13 +25,21 @@ HIP_FLAGS = [
 ]
 
 
+# Per-file additions.  eval16.hip: let the MFMAs accumulate in VGPRs (the
+# epilogue then reads the accumulators without v_accvgpr_read copies).
+FILE_FLAGS = {
+    "eval16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def _compile(src):
     obj = osp.join(OBJ, osp.basename(src) + ".o")
     deps = [src] + [osp.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
     deps.append(osp.join(HERE, "..", "include", "mplicits.h"))
+    deps.append(osp.abspath(__file__))  # flags live here
     if osp.exists(obj) and all(osp.getmtime(obj) >= osp.getmtime(d) for d in deps):
         return obj, ""
-    cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + HIP_FLAGS + FILE_FLAGS.get(osp.basename(src), []) + ["-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{p.stdout}\n{p.stderr}")

@@ -9,19 +9,21 @@
 //    straight into layer 1's B operand;
 //  - the activations of a layer that feeds another hidden layer are handed over
 //    through the wave's own LDS region instead of a second register buffer, so
-//    the VGPRs hold one B operand set, a 16-deep A-fragment prefetch that runs
-//    continuously across layer boundaries, and two accumulator sets;
+//    the VGPRs hold one B operand set and two accumulator sets;
+//  - the A fragments (weights) of all hidden layers form one continuous stream.
+//    For widths >= 128 the four waves of a workgroup copy it, 16 KB stage by
+//    16 KB stage, into a two-slot LDS ring with direct global->LDS loads issued
+//    a whole stage ahead (one barrier per stage), and each wave reads its
+//    fragments from LDS a few steps ahead; width 64 streams from global memory
+//    through a register ring;
 //  - the epilogue of row block rb (bias is the MFMA C operand; sine, packing,
 //    LDS store or output dot product) is issued in slices between the MFMAs of
 //    row block rb + 1, so the transcendental unit and the matrix core overlap.
 #include "eval_dispatch.h"
 
-// A-fragment prefetch depth of k_level16 (fragments in flight per wave).
-#ifndef MP_PF16
-#define MP_PF16 8
-#endif
-
 namespace mp {
+
+typedef __attribute__((address_space(3))) void lds_void;
 
 template <class S, int W, int NC>
 struct Lvl16 {
@@ -31,14 +33,90 @@ struct Lvl16 {
   static constexpr int RBS = W / 32;            // 32-row blocks per layer
   static constexpr int KBS = W / 16;            // 16-deep K blocks per layer
   static constexpr int STEPS = RBS * KBS;       // A fragments per layer
-  static constexpr int PF = STEPS < MP_PF16 ? STEPS : MP_PF16;
-  static_assert(STEPS % PF == 0, "the prefetch ring must tile a layer");
-  static_assert((size_t)PF * 64 * sizeof(Frag) <= APK_PAD, "prefetch runs past the padding");
   static constexpr int TILE = NC * 32;          // points per wave
-  static constexpr int LDS_FRAGS = NC * KBS * 64;  // per wave, in fragments
+  static constexpr int HANDOFF_FRAGS = NC * KBS * 64;  // per wave, in fragments
+
+  // A stream.  Staged: stages of SF fragments, two LDS slots, read D steps
+  // ahead.  The slot of a step must not depend on the (run-time) layer index,
+  // so a layer spans an even number of stages.
+  static constexpr bool STAGED = STEPS % 32 == 0;
+  static constexpr int SF = 16;
+  static constexpr int D = 4;
+  static_assert(!STAGED || STEPS % D == 0, "the read-ahead ring must tile a layer");
+  static constexpr int RING_FRAGS = STAGED ? 2 * SF * 64 : 0;
+  static constexpr int PF = STAGED ? D : (STEPS < 8 ? STEPS : 8);
+  static_assert(STAGED || STEPS % PF == 0, "the prefetch ring must tile a layer");
+  static_assert((size_t)(STAGED ? 2 * SF : PF) * 64 * sizeof(Frag) <= APK_PAD,
+                "the stream reads past the padding");
+  static constexpr size_t LDS_BYTES = (size_t)(RING_FRAGS + 4 * HANDOFF_FRAGS) * sizeof(Frag);
+
   // accumulator registers retired per K step by the pipelined epilogue
   static constexpr int QPK = KBS >= 16 ? 1 : 16 / KBS;
   static constexpr int KPQ = KBS >= 16 ? KBS / 16 : 1;
+
+  struct Stream {
+    // A is deliberately not __restrict__: the empty asm with a memory clobber
+    // after every step pins each look-ahead load to its step (through a
+    // restrict pointer the loads may sink to their uses and expose their
+    // latency).
+    const Frag* A;     // hidden-layer fragments, layer-major, APK_PAD readable bytes after the end
+    Frag* ring;        // LDS ring (staged)
+    int lane, wave;
+    Frag pf[PF];
+
+    // Stage `stage` of the stream into ring slot `slot`: each wave copies 4 of
+    // the stage's 16 fragments (1 KB each, one 16-byte direct load per lane).
+    __device__ inline void copy_stage(int64_t stage, int slot) {
+#pragma unroll
+      for (int i = 0; i < SF / 4; ++i) {
+        const int f = (SF / 4) * wave + i;
+        __builtin_amdgcn_global_load_lds((const void*)(A + (stage * SF + f) * 64 + lane),
+                                         (lds_void*)(ring + (slot * SF + f) * 64), 16, 0, 0);
+      }
+    }
+    // Make the copies issued so far visible to the whole workgroup.
+    __device__ inline void barrier() {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+    }
+
+    // Before layer 0: start the first two stages (or fill the register ring).
+    __device__ inline void start() {
+      if constexpr (STAGED) {
+        copy_stage(0, 0);
+        copy_stage(1, 1);
+      } else {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) pf[d] = A[d * 64 + lane];
+      }
+    }
+    // After layer 0: stage 0 visible, first D fragments in registers.
+    __device__ inline void begin() {
+      if constexpr (STAGED) {
+        barrier();
+#pragma unroll
+        for (int d = 0; d < D; ++d) pf[d] = ring[d * 64 + lane];
+      }
+    }
+    // Fragment of step s of hidden layer j; issues the look-ahead.
+    __device__ inline Frag next(int j, int s) {
+      if constexpr (STAGED) {
+        const int t = s + D;  // step whose fragment is read now
+        if (t % SF == 0) {
+          // every wave is past its last read of the slot about to be refilled
+          barrier();
+          copy_stage((int64_t)j * (STEPS / SF) + t / SF + 1, (t / SF + 1) % 2);
+        }
+        const Frag a = pf[s % D];
+        pf[s % D] = ring[((t / SF) % 2 * SF + t % SF) * 64 + lane];
+        return a;
+      } else {
+        const Frag a = pf[s % PF];
+        pf[s % PF] = A[((size_t)j * STEPS + s + PF) * 64 + lane];
+        return a;
+      }
+    }
+  };
 
   // Element q of a 32x32 accumulator, as an operand index of the next layer.
   static __device__ inline void put(Frag (&b)[NC][KBS], int c, int rb, int q, float v) {
@@ -93,18 +171,12 @@ struct Lvl16 {
     }
   }
 
-  // One hidden GEMM over all row blocks with the epilogue of block rb - 1
-  // issued in slices between the MFMAs of block rb.  A points at this layer's
-  // fragments; the prefetch ring pf continues into the next layer.
-  // epi(rb, q, acc) retires accumulator register q of row block rb.
+  // Hidden GEMM j over all row blocks with the epilogue of block rb - 1 issued
+  // in slices between the MFMAs of block rb.  epi(rb, q, acc) retires
+  // accumulator register q of row block rb.
   template <class Epi>
-  // A is deliberately not __restrict__: the empty asm with a memory clobber
-  // after every step must pin each prefetch load to its step (through a
-  // restrict pointer the loads may sink to their uses and expose their
-  // latency).
-  static __device__ inline void gemm(const Frag* A, const float* __restrict__ bc,
-                                     Frag (&pf)[PF], const Frag (&b)[NC][KBS], int lane, int g,
-                                     Epi epi) {
+  static __device__ inline void gemm(Stream& A, int j, const float* __restrict__ bc,
+                                     const Frag (&b)[NC][KBS], int g, Epi epi) {
     Acc acc[2][NC];
     Acc bias;
     load16(bc + g * 16, bias);
@@ -113,9 +185,7 @@ struct Lvl16 {
       Acc bias_next;
 #pragma unroll
       for (int kb = 0; kb < KBS; ++kb) {
-        const int s = rb * KBS + kb;
-        const Frag a = pf[s % PF];
-        pf[s % PF] = A[(size_t)(s + PF) * 64 + lane];
+        const Frag a = A.next(j, rb * KBS + kb);
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           acc[rb & 1][c] = S::mma(a, b[c][kb], kb == 0 ? bias : acc[rb & 1][c]);
@@ -132,12 +202,14 @@ struct Lvl16 {
     for (int q = 0; q < 16; ++q) epi(RBS - 1, q, acc[(RBS - 1) & 1]);
   }
 
-  // f_level(x) for the NC points of this lane (every lane group returns the
-  // same value for its column).  lds: this wave's handoff region.
+  // f_level(x) for the NC points of this lane (both lane groups return the
+  // same value for their column).  lds: the workgroup's LDS (ring first, then
+  // one handoff region per wave).  Every wave of the workgroup must call it.
   static __device__ inline void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
-                                     Frag* __restrict__ lds) {
+                                     Frag* lds) {
     const int lane = threadIdx.x & 63;
     const int g = lane / 32;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const float bl = *L.blp;
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = 0.f;
@@ -149,40 +221,42 @@ struct Lvl16 {
       for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
       return;
     }
-    const Frag* A = reinterpret_cast<const Frag*>(L.apk);
-    Frag pf[PF];
-#pragma unroll
-    for (int d = 0; d < PF; ++d) pf[d] = A[d * 64 + lane];
+    Frag* hand = lds + RING_FRAGS + wave * HANDOFF_FRAGS;
+    Stream A{reinterpret_cast<const Frag*>(L.apk), lds, lane, wave, {}};
+    A.start();
     first(L, x, b, lane, g, nullptr, f, false);
+    A.begin();
 
     // hidden layers that feed another hidden layer: B registers -> LDS -> B registers
     for (int j = 0; j + 1 < n_gemm; ++j) {
       Frag st[NC];
-      gemm(A + (size_t)j * STEPS * 64, L.biasc + (size_t)j * W, pf, b, lane, g,
-           [&](int rb, int q, const Acc (&acc)[NC]) {
+      gemm(A, j, L.biasc + (size_t)j * W, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
 #pragma unroll
-             for (int c = 0; c < NC; ++c) {
-               S::set(st[c], q % 8, sin_rev(acc[c][q]));
-               if (q % 8 == 7) lds[(c * KBS + 2 * rb + q / 8) * 64 + lane] = st[c];
-             }
-           });
+        for (int c = 0; c < NC; ++c) {
+          S::set(st[c], q % 8, sin_rev(acc[c][q]));
+          if (q % 8 == 7) hand[(c * KBS + 2 * rb + q / 8) * 64 + lane] = st[c];
+        }
+      });
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int kb = 0; kb < KBS; ++kb) b[c][kb] = lds[(c * KBS + kb) * 64 + lane];
+        for (int kb = 0; kb < KBS; ++kb) b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
     }
 
     // last hidden layer fused with the output layer (output weights of row
     // block rb + 1 are fetched while rb retires)
     Acc wl, wl_next;
     load16(L.wlc + g * 16, wl);
-    gemm(A + (size_t)(n_gemm - 1) * STEPS * 64, L.biasc + (size_t)(n_gemm - 1) * W, pf, b, lane, g,
+    gemm(A, n_gemm - 1, L.biasc + (size_t)(n_gemm - 1) * W, b, g,
          [&](int rb, int q, const Acc (&acc)[NC]) {
            if (q == 0 && rb + 1 < RBS) load16(L.wlc + ((rb + 1) * 2 + g) * 16, wl_next);
 #pragma unroll
            for (int c = 0; c < NC; ++c) f[c] = fmaf(wl[q], sin_rev(acc[c][q]), f[c]);
            if (q == 15 && rb + 1 < RBS) wl = wl_next;
          });
+    // the look-ahead copies past the end land in the ring; let them finish
+    // before the workgroup can retire and its LDS be reassigned
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
   }
@@ -195,12 +269,12 @@ __global__ __launch_bounds__(256) void k_level16(LevelDev L, EvalTarget t) {
   typedef Lvl16<S, W, NC> M;
   typedef typename S::Frag Frag;
   extern __shared__ uint4 lds_raw[];
-  Frag* lds_frags = reinterpret_cast<Frag*>(lds_raw);
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int64_t base = wave * M::TILE;
   const int64_t n = (t.src == SRC_LIST) ? (int64_t)*t.list_count : t.n;
-  if (base >= n) return;  // wave-uniform; no block-level synchronisation follows
+  // Workgroup-uniform exit: the waves of a workgroup share barriers, so a wave
+  // whose tile lies past n still runs (on clamped points) and stores nothing.
+  if ((int64_t)blockIdx.x * 4 * M::TILE >= n) return;
+  const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * M::TILE;
 
   float x[NC][3];
   int64_t key[NC];
@@ -221,7 +295,7 @@ __global__ __launch_bounds__(256) void k_level16(LevelDev L, EvalTarget t) {
     }
   }
   float f[NC];
-  M::eval(L, x, f, lds_frags + (threadIdx.x >> 6) * M::LDS_FRAGS);
+  M::eval(L, x, f, reinterpret_cast<Frag*>(lds_raw));
 
   // Lanes of group 0 own the store (both groups hold the same value).
 #pragma unroll
@@ -256,21 +330,20 @@ static int launch16_typed(const LevelDev& L, const EvalTarget& t, hipStream_t st
   const int64_t blocks = (t.n + per_block - 1) / per_block;
   if (blocks <= 0) return MP_OK;
   MP_REQUIRE(blocks < (1ll << 31), MP_EINVAL, "level launch: %lld points is too many", (long long)t.n);
-  const size_t lds = (size_t)4 * M::LDS_FRAGS * sizeof(typename S::Frag);
+  static_assert(M::LDS_BYTES <= 160 * 1024, "LDS budget of one workgroup");
   static bool attr_set = false;
   if (!attr_set) {
     MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level16<S, W, NC>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)M::LDS_BYTES));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_level16<S, W, NC>), dim3((unsigned)blocks), dim3(256), lds, stream, L, t);
+  hipLaunchKernelGGL((k_level16<S, W, NC>), dim3((unsigned)blocks), dim3(256), M::LDS_BYTES, stream, L, t);
   MP_LAUNCHED("k_level16");
   return MP_OK;
 }
 
-// Column blocks per wave: one 32-point block per 512 / W of activation width
-// (the B operand then takes 128 VGPRs), or one when the stack asks for
-// narrow tiles.
+// Column blocks per wave: the B operand takes NC * W / 4 VGPRs (128 at the
+// defaults), or one block when the stack asks for narrow tiles.
 template <class S>
 static int launch16_width(const LevelDev& L, int nc, const EvalTarget& t, hipStream_t stream) {
   if (nc == 1) {

@@ -39,7 +39,7 @@ inline int padded_width(int w) {
 
 // Bytes of readable padding after the packed hidden-layer A fragments: the
 // level kernels prefetch a fixed number of fragments ahead of their use.
-constexpr size_t APK_PAD = 16 * 64 * 16;
+constexpr size_t APK_PAD = 32 * 64 * 16;
 
 // Launch-shape selection shared by every kernel family (a tile-shape override
 // from the autotuner is applied at stack creation).
