@@ -25,10 +25,10 @@ HIP_FLAGS = [
 ]
 
 
-# Per-file additions.  eval16.hip: let the MFMAs accumulate in VGPRs (the
+# Per-file additions.  level.hip: let the MFMAs accumulate in VGPRs (the
 # epilogue then reads the accumulators without v_accvgpr_read copies).
 FILE_FLAGS = {
-    "eval16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "level.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
 }
 
 

@@ -27,8 +27,6 @@ struct EvalTarget {
 
 int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream);
 
-// launch_level for the 32x32x16 BF16 / FP16 shapes (k_level16).
-int launch_level16(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream);
 
 // f_level and its input gradient at n points xyz [n][3]; f may be NULL;
 // accumulate adds into f / g instead of overwriting.
@@ -39,7 +37,5 @@ int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n
 int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
                 hipStream_t stream);
 
-// Points handled by one wave of the level kernels.
-int level_tile(const mp_stack* st, int level);
 
 }  // namespace mp

@@ -1,0 +1,307 @@
+// One SIREN level evaluated by one wave on the matrix cores, laid out for one
+// wave per SIMD (i3d/model.py:87-129 with the frequency baked in).
+//
+// Chained-layer scheme (shared with mlp_core.h): the hidden GEMMs run
+// transposed, A = W_i (pre-packed fragments, streamed), B = activations of the
+// tile's NC*ROWS points held in VGPRs; accumulator register q of row block rb
+// is reduction index t = CREGS*rb + q of the next layer, i.e. element t % EPL
+// of its B fragment t / EPL (the packer permutes the next layer's columns to
+// match).
+//
+// Differences from the Mlp chain:
+//  - 16-bit shapes: layer 0 (3 -> width) runs on the FP32 matrix cores, two
+//    32x32x2 MFMAs per 32-row block over (x, y, z, 1) (weights pre-scaled to
+//    revolutions); FP32 shape: range-reduced FMA + sine on the VALU, written
+//    straight into the B registers (its accuracy is the FP32 path's);
+//  - a layer that feeds another hidden layer hands its output over through the
+//    wave's own LDS region (B registers -> LDS -> B registers) instead of a
+//    second register set, leaving VGPRs for a deep A look-ahead and two
+//    accumulator sets;
+//  - the A fragments of all hidden layers form one continuous stream.  16-bit
+//    shapes at width >= 128 (when the caller allows workgroup barriers): the
+//    four waves of a workgroup copy it, 16 KB stage by 16 KB stage, into a
+//    two-slot LDS ring with direct global->LDS loads a whole stage ahead and
+//    read it back a few steps ahead.  Otherwise: a register ring of PF
+//    fragments loaded from global memory, continuous across layers;
+//  - the epilogue of row block rb (bias is the MFMA C operand; sine, packing,
+//    LDS store or output dot product) is issued in slices between the MFMAs
+//    of row block rb + 1, so the transcendental unit and the matrix core
+//    overlap as far as one wave's in-order issue allows.
+#pragma once
+
+#include "stack.h"
+
+namespace mp {
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <class S, int W, int NC, bool ALLOW_STAGED>
+struct LevelCore {
+  typedef typename S::Frag Frag;
+  typedef typename S::Acc Acc;
+  static constexpr int ROWS = S::ROWS, KB = S::KB, CREGS = S::CREGS, EPL = S::EPL, GROUPS = S::GROUPS;
+  static constexpr bool MFMA_FIRST = S::FIRST_IN_REVS;  // 16-bit shapes (32-row C layout)
+  static_assert(!MFMA_FIRST || ROWS == 32, "the MFMA first layer uses the 32x32 C layout");
+  static constexpr int RBS = W / ROWS;           // row blocks per layer
+  static constexpr int KBS = W / KB;             // K blocks per layer
+  static constexpr int STEPS = RBS * KBS;        // A fragments per layer
+  static constexpr int TILE = NC * ROWS;         // points per wave
+  static constexpr int HANDOFF_FRAGS = NC * KBS * 64;  // per wave, in fragments
+
+  static constexpr bool STAGED = ALLOW_STAGED && sizeof(Frag) == 16 && STEPS % 32 == 0;
+  static constexpr int SF = 16;  // fragments per stage
+  static constexpr int D = 4;    // LDS read-ahead (steps)
+  static constexpr int RING_FRAGS = STAGED ? 2 * SF * 64 : 0;
+  static constexpr int PF_REG = sizeof(Frag) == 16 ? 8 : 32;
+  static constexpr int PF = STAGED ? D : (STEPS < PF_REG ? STEPS : PF_REG);
+  static_assert(STEPS % PF == 0, "the look-ahead ring must tile a layer");
+  static_assert((size_t)(STAGED ? 2 * SF : PF) * 64 * sizeof(Frag) <= APK_PAD,
+                "the stream reads past the padding");
+  // LDS of a workgroup of four waves: ring, then one handoff region per wave
+  static constexpr size_t LDS_BYTES = (size_t)(RING_FRAGS + 4 * HANDOFF_FRAGS) * sizeof(Frag);
+
+  // epilogue slices: accumulator registers retired per K step
+  static constexpr int QPK = KBS >= CREGS ? 1 : CREGS / KBS;
+  static constexpr int KPQ = KBS >= CREGS ? KBS / CREGS : 1;
+
+  struct Stream {
+    // A is deliberately not __restrict__: the empty asm with a memory clobber
+    // after every step pins each look-ahead load to its step (through a
+    // restrict pointer the loads may sink to their uses and expose their
+    // latency).
+    const Frag* A;  // hidden-layer fragments, layer-major, APK_PAD readable bytes after the end
+    Frag* ring;     // LDS ring (staged)
+    int lane, wave;
+    Frag pf[PF];
+
+    // Stage `stage` into ring slot `slot`: each of the four waves copies 4 of
+    // the stage's 16 fragments (one 16-byte direct load per lane each).
+    __device__ inline void copy_stage(int64_t stage, int slot) {
+#pragma unroll
+      for (int i = 0; i < SF / 4; ++i) {
+        const int f = (SF / 4) * wave + i;
+        __builtin_amdgcn_global_load_lds((const void*)(A + (stage * SF + f) * 64 + lane),
+                                         (lds_void*)(ring + (slot * SF + f) * 64), 16, 0, 0);
+      }
+    }
+    __device__ inline void barrier() {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+    }
+    __device__ inline void start() {
+      if constexpr (STAGED) {
+        copy_stage(0, 0);
+        copy_stage(1, 1);
+      } else {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) pf[d] = A[d * 64 + lane];
+      }
+    }
+    __device__ inline void begin() {
+      if constexpr (STAGED) {
+        barrier();
+#pragma unroll
+        for (int d = 0; d < D; ++d) pf[d] = ring[d * 64 + lane];
+      }
+    }
+    // Fragment of step s of hidden layer j; issues the look-ahead.
+    __device__ inline Frag next(int j, int s) {
+      if constexpr (STAGED) {
+        const int t = s + D;  // step whose fragment is read now
+        if (t % SF == 0) {
+          // every wave is past its last read of the slot about to be refilled
+          barrier();
+          copy_stage((int64_t)j * (STEPS / SF) + t / SF + 1, (t / SF + 1) % 2);
+        }
+        const Frag a = pf[s % D];
+        pf[s % D] = ring[((t / SF) % 2 * SF + t % SF) * 64 + lane];
+        return a;
+      } else {
+        const Frag a = pf[s % PF];
+        pf[s % PF] = A[((size_t)j * STEPS + s + PF) * 64 + lane];
+        return a;
+      }
+    }
+    // Outstanding look-ahead copies land in the ring; let them finish before
+    // the workgroup can retire and its LDS be reassigned.
+    __device__ inline void finish() {
+      if constexpr (STAGED) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+  };
+
+  static __device__ inline void loadc(const float* __restrict__ p, Acc& v) {
+    if constexpr (CREGS % 4 == 0) {
+      const float4* p4 = reinterpret_cast<const float4*>(p);
+#pragma unroll
+      for (int i = 0; i < CREGS / 4; ++i) {
+        const float4 t = p4[i];
+        v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < CREGS; ++q) v[q] = p[q];
+    }
+  }
+
+  static __device__ inline float group_sum(float v) {
+    if (GROUPS == 4) v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+  }
+
+  // Layer 0 into the B registers of layer 1.
+  static __device__ inline void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][KBS],
+                                      int lane, int g) {
+    if constexpr (MFMA_FIRST) {
+      float bxy[NC], bz1[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        bxy[c] = g == 0 ? x[c][0] : x[c][1];
+        bz1[c] = g == 0 ? x[c][2] : 1.f;
+      }
+      Acc zero;
+#pragma unroll
+      for (int q = 0; q < CREGS; ++q) zero[q] = 0.f;
+#pragma unroll
+      for (int rb = 0; rb < RBS; ++rb) {
+        const float2 a = L.w0m[rb * 64 + lane];
+        Acc acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, bxy[c], zero, 0, 0, 0);
+          acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, bz1[c], acc[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < CREGS; ++q) {
+          const int t = rb * CREGS + q;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) S::set(b[c][t / EPL], t % EPL, sin_rev(acc[c][q]));
+        }
+      }
+    } else {
+      // B element e of K block kb holds virtual unit v = KB kb + EPL g + e
+#pragma unroll
+      for (int kb = 0; kb < KBS; ++kb) {
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+          const float4 w = L.w0v[kb * KB + g * EPL + e];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const float a = fmaf(w.z, x[c][2], fmaf(w.y, x[c][1], fmaf(w.x, x[c][0], w.w)));
+            S::set(b[c][kb], e, sin_reduced(a));
+          }
+        }
+      }
+    }
+  }
+
+  // A level with one sine layer: f = w_L . sin(W_0 x + b_0) on the VALU.
+  static __device__ inline void single(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC], int g) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) f[c] = 0.f;
+    for (int v0 = 0; v0 < W; v0 += KB) {
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) {
+        const int v = v0 + g * EPL + e;
+        const float4 w = L.w0v[v];
+        const float wl = L.wlv[v];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const float a = fmaf(w.z, x[c][2], fmaf(w.y, x[c][1], fmaf(w.x, x[c][0], w.w)));
+          f[c] = fmaf(wl, MFMA_FIRST ? sin_rev(a) : sin_reduced(a), f[c]);
+        }
+      }
+    }
+  }
+
+  // Hidden GEMM j over all row blocks with the epilogue of block rb - 1 issued
+  // in slices between the MFMAs of block rb.  epi(rb, q, acc) retires
+  // accumulator register q of row block rb.
+  template <class Epi>
+  static __device__ inline void gemm(Stream& A, int j, const float* __restrict__ bc,
+                                     const Frag (&b)[NC][KBS], int g, Epi epi) {
+    Acc acc[2][NC];
+    Acc bias;
+    loadc(bc + g * CREGS, bias);
+#pragma unroll
+    for (int rb = 0; rb < RBS; ++rb) {
+      Acc bias_next;
+#pragma unroll
+      for (int kb = 0; kb < KBS; ++kb) {
+        const Frag a = A.next(j, rb * KBS + kb);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          acc[rb & 1][c] = S::mma(a, b[c][kb], kb == 0 ? bias : acc[rb & 1][c]);
+        if (kb == 0 && rb + 1 < RBS) loadc(bc + ((rb + 1) * GROUPS + g) * CREGS, bias_next);
+        if (rb > 0 && kb % KPQ == 0) {
+#pragma unroll
+          for (int i = 0; i < QPK; ++i) epi(rb - 1, (kb / KPQ) * QPK + i, acc[(rb - 1) & 1]);
+        }
+        asm volatile("" ::: "memory");
+      }
+      if (rb + 1 < RBS) bias = bias_next;
+    }
+#pragma unroll
+    for (int q = 0; q < CREGS; ++q) epi(RBS - 1, q, acc[(RBS - 1) & 1]);
+  }
+
+  // f_level(x) for the NC points of this lane (every lane group returns the
+  // same value for its column).  lds: the workgroup's LDS (ring, then one
+  // handoff region per wave).  With the staged stream every wave of the
+  // workgroup must call it.
+  static __device__ inline void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
+                                     Frag* lds) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane / ROWS;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const float bl = *L.blp;
+    const int n_gemm = L.n_hidden - 1;
+    if (n_gemm == 0) {
+      single(L, x, f, g);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
+      return;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) f[c] = 0.f;
+    Frag b[NC][KBS];
+    Frag* hand = lds + RING_FRAGS + wave * HANDOFF_FRAGS;
+    Stream A{reinterpret_cast<const Frag*>(L.apk), lds, lane, wave, {}};
+    A.start();
+    first(L, x, b, lane, g);
+    A.begin();
+
+    // hidden layers that feed another hidden layer: B registers -> LDS -> B registers
+    for (int j = 0; j + 1 < n_gemm; ++j) {
+      Frag st[NC];
+      gemm(A, j, L.biasc + (size_t)j * W, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
+        const int t = rb * CREGS + q;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          S::set(st[c], t % EPL, sin_rev(acc[c][q]));
+          if (t % EPL == EPL - 1) hand[(c * KBS + t / EPL) * 64 + lane] = st[c];
+        }
+      });
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int kb = 0; kb < KBS; ++kb) b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
+    }
+
+    // last hidden layer fused with the output layer (output weights of row
+    // block rb + 1 are fetched while rb retires)
+    Acc wl, wl_next;
+    loadc(L.wlc + g * CREGS, wl);
+    gemm(A, n_gemm - 1, L.biasc + (size_t)(n_gemm - 1) * W, b, g,
+         [&](int rb, int q, const Acc (&acc)[NC]) {
+           if (q == 0 && rb + 1 < RBS) loadc(L.wlc + ((rb + 1) * GROUPS + g) * CREGS, wl_next);
+#pragma unroll
+           for (int c = 0; c < NC; ++c) f[c] = fmaf(wl[q], sin_rev(acc[c][q]), f[c]);
+           if (q == CREGS - 1 && rb + 1 < RBS) wl = wl_next;
+         });
+    A.finish();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
+  }
+};
+
+}  // namespace mp
