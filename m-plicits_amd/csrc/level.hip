@@ -38,7 +38,8 @@ __global__ __launch_bounds__(256) void k_level(LevelDev L, EvalTarget t) {
     }
   }
   float f[NC];
-  M::eval(L, x, f, reinterpret_cast<Frag*>(lds_raw));
+  Frag* lds = reinterpret_cast<Frag*>(lds_raw);
+  M::eval(L, x, f, lds, lds + M::RING_FRAGS + (threadIdx.x >> 6) * M::HANDOFF_FRAGS);
 
   // Lanes of group 0 own the store (every group holds the same value).
 #pragma unroll

@@ -245,11 +245,11 @@ struct LevelCore {
   }
 
   // f_level(x) for the NC points of this lane (every lane group returns the
-  // same value for its column).  lds: the workgroup's LDS (ring, then one
-  // handoff region per wave).  With the staged stream every wave of the
-  // workgroup must call it.
+  // same value for its column).  ring: the workgroup's staging ring (staged
+  // stream only; then every wave of the workgroup must call it); hand: this
+  // wave's handoff region of HANDOFF_FRAGS fragments.
   static __device__ inline void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
-                                     Frag* lds) {
+                                     Frag* ring, Frag* hand) {
     const int lane = threadIdx.x & 63;
     const int g = lane / ROWS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -264,8 +264,7 @@ struct LevelCore {
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = 0.f;
     Frag b[NC][KBS];
-    Frag* hand = lds + RING_FRAGS + wave * HANDOFF_FRAGS;
-    Stream A{reinterpret_cast<const Frag*>(L.apk), lds, lane, wave, {}};
+    Stream A{reinterpret_cast<const Frag*>(L.apk), ring, lane, wave, {}};
     A.start();
     first(L, x, b, lane, g);
     A.begin();

@@ -20,6 +20,7 @@
 // Rays start where they enter the sphere of radius bound_radius
 // (InitializeRays_kernel, kernels.cu:400-487).
 #include "eval_dispatch.h"
+#include "level_core.h"
 #include "trace.h"
 
 namespace mp {
@@ -29,14 +30,20 @@ __global__ void k_fill_ull(unsigned long long* p, int n, unsigned long long v) {
   if (i < n) p[i] = v;
 }
 
+// LDS handoff bytes per wave: every width the tracer runs (NC * width <= 512)
+// fits this region.
+constexpr size_t TRACE_HAND_BYTES = 32 * 1024;
+
+// f_level at the NC points of this lane; hand: the wave's handoff region.
 template <class S, int NC>
-__device__ inline void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC]) {
+__device__ inline void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
+                                      typename S::Frag* hand) {
   switch (L.width) {
-    case 64: Mlp<S, 64, NC>::eval(L, x, f); break;
-    case 128: Mlp<S, 128, NC>::eval(L, x, f); break;
-    case 256: Mlp<S, 256, NC>::eval(L, x, f); break;
+    case 64: LevelCore<S, 64, NC, false>::eval(L, x, f, nullptr, hand); break;
+    case 128: LevelCore<S, 128, NC, false>::eval(L, x, f, nullptr, hand); break;
+    case 256: LevelCore<S, 256, NC, false>::eval(L, x, f, nullptr, hand); break;
     default:
-      if constexpr (NC == 1) Mlp<S, 512, 1>::eval(L, x, f);
+      if constexpr (NC == 1) LevelCore<S, 512, 1, false>::eval(L, x, f, nullptr, hand);
       break;
   }
 }
@@ -83,6 +90,9 @@ __device__ inline void ray_init(const TraceArgs& a, int64_t rid, float o[3], flo
 
 template <class S, int NC>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+  extern __shared__ uint4 lds_raw[];
+  typename S::Frag* hand = reinterpret_cast<typename S::Frag*>(
+      reinterpret_cast<char*>(lds_raw) + (threadIdx.x >> 6) * TRACE_HAND_BYTES);
   const int lane = threadIdx.x & 63;
   const int col = lane % S::ROWS;
   const int64_t total = a.total_rays;
@@ -146,27 +156,33 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
       x[c][1] = act ? p[c][1] : 0.f;
       x[c][2] = act ? p[c][2] : 0.f;
     }
-    eval_level_any<S, NC>(a.levels[0], x, F);
+    // one call site for every level keeps a single copy of each width's code
+    for (int l = 0; l < a.n_levels; ++l) {
+      if (l > 0) {
+        bool any = false;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      Fsum[c] = F[c];
-      need[c] = rid[c] >= 0 && a.n_levels > 1 && F[c] < a.band_scale * a.deltas[0];
-      if (lane < S::ROWS && rid[c] >= 0) evals[0]++;
-    }
-    for (int l = 1; l < a.n_levels; ++l) {
-      bool any = false;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) any |= need[c];
-      if (__ballot(any) == 0) break;
+        for (int c = 0; c < NC; ++c) any |= need[c];
+        if (__ballot(any) == 0) break;
+      }
       float r[NC];
-      eval_level_any<S, NC>(a.levels[l], x, r);
+      eval_level_any<S, NC>(a.levels[l], x, r, hand);
+      if (l == 0) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        if (need[c]) {
-          Fsum[c] += r[c];
-          if (lane < S::ROWS) evals[l]++;
+        for (int c = 0; c < NC; ++c) {
+          F[c] = r[c];
+          Fsum[c] = r[c];
+          need[c] = rid[c] >= 0 && a.n_levels > 1 && r[c] < a.band_scale * a.deltas[0];
+          if (lane < S::ROWS && rid[c] >= 0) evals[0]++;
         }
-        need[c] = need[c] && l + 1 < a.n_levels && fabsf(Fsum[c]) < a.band_scale * a.deltas[l];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          if (need[c]) {
+            Fsum[c] += r[c];
+            if (lane < S::ROWS) evals[l]++;
+          }
+          need[c] = need[c] && l + 1 < a.n_levels && fabsf(Fsum[c]) < a.band_scale * a.deltas[l];
+        }
       }
     }
     // step, hit and exit tests
@@ -278,6 +294,19 @@ __global__ void k_shade(TraceArgs a, const uint32_t* ids, const float* grad, int
   }
 }
 
+template <class S, int NC>
+static int launch_trace(int blocks, size_t lds, hipStream_t s, const TraceArgs& a) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trace<S, NC>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_trace<S, NC>), dim3(blocks), dim3(256), lds, s, a);
+  MP_LAUNCHED("k_trace");
+  return MP_OK;
+}
+
 }  // namespace mp
 
 using namespace mp;
@@ -342,22 +371,24 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   MP_HIP(hipGetDevice(&dev));
   MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int nc = (st->nc == 2 && maxw <= 256) ? 2 : 1;
-  // persistent grid: enough waves to fill every SIMD twice over
-  const int blocks = cus * 2;
+  // persistent grid: one workgroup of four waves per CU (the LDS handoff
+  // regions take 128 KB of the CU's LDS)
+  const int blocks = cus;
+  const size_t lds = 4 * TRACE_HAND_BYTES;
   switch (st->shape) {
     case SH_32x32x16_BF16:
-      if (nc == 2) hipLaunchKernelGGL((k_trace<S32x32x16_bf16, 2>), dim3(blocks), dim3(256), 0, s, a);
-      else hipLaunchKernelGGL((k_trace<S32x32x16_bf16, 1>), dim3(blocks), dim3(256), 0, s, a);
+      rc = nc == 2 ? launch_trace<S32x32x16_bf16, 2>(blocks, lds, s, a)
+                   : launch_trace<S32x32x16_bf16, 1>(blocks, lds, s, a);
       break;
     case SH_32x32x16_F16:
-      if (nc == 2) hipLaunchKernelGGL((k_trace<S32x32x16_f16, 2>), dim3(blocks), dim3(256), 0, s, a);
-      else hipLaunchKernelGGL((k_trace<S32x32x16_f16, 1>), dim3(blocks), dim3(256), 0, s, a);
+      rc = nc == 2 ? launch_trace<S32x32x16_f16, 2>(blocks, lds, s, a)
+                   : launch_trace<S32x32x16_f16, 1>(blocks, lds, s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_trace<S16x16x4_f32, 1>), dim3(blocks), dim3(256), 0, s, a);
+      rc = launch_trace<S16x16x4_f32, 1>(blocks, lds, s, a);
       break;
   }
-  MP_LAUNCHED("k_trace");
+  if (rc != MP_OK) return rc;
   hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, depth,
                      hit_pos, total, ids, cpos, &ctr[8]);
   MP_LAUNCHED("k_collect_hits");
