@@ -101,6 +101,33 @@ int mp_eval(const mp_stack* stack, const float* xyz, int64_t n, int lvl0, int lv
 int mp_extract_volume(mp_stack* stack, int N, int grid_mode, int64_t chunk, const float* deltas,
                       int n_deltas, float* vol, int64_t* level_counts, void* stream);
 
+/* Batched band-culled extraction of many stacks (uneven widths, levels and
+ * resolutions; every stack in the same precision).  The reference extracts
+ * one model per create_mesh_multistage call (i3d/meshing.py:135-264); this
+ * entry point runs the same per-model semantics for all models with one
+ * persistent launch per stage.  mp_extract_batch_plan allocates the device
+ * work lists and uploads the job tables (host-sync point); mp_extract_batch_run
+ * only launches kernels (no allocation, host synchronisation or memset), so a
+ * caller may capture it in a hipGraph and replay it; mp_extract_batch_counts
+ * reads back the per-job, per-level sample counts of the last run (host-sync
+ * point, [n_jobs][MP_MAX_LEVELS]).  The plan keeps pointers to the stacks'
+ * packed weights and to the volumes: both must outlive it. */
+typedef struct mp_batch mp_batch;
+typedef struct mp_extract_job {
+  mp_stack* stack;
+  int N;                /* grid resolution */
+  int grid_mode;        /* MP_GRID_* */
+  int64_t chunk;        /* culling chunk (max_batch) */
+  int n_deltas;         /* 0 (no culling) or stack levels - 1 */
+  float deltas[MP_MAX_LEVELS - 1];
+  float* vol;           /* N^3 floats, device */
+} mp_extract_job;
+
+int mp_extract_batch_plan(const mp_extract_job* jobs, int n_jobs, void* stream, mp_batch** out);
+int mp_extract_batch_run(mp_batch* plan, void* stream);
+int mp_extract_batch_counts(const mp_batch* plan, int64_t* counts, void* stream);
+void mp_extract_batch_destroy(mp_batch* plan);
+
 /* Sample coordinates of grid index range [first, first+n) into xyz [n][3]
  * (gen_mc_coordinate_grid, i3d/meshing.py:11-69, columns 0..2). */
 int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, void* stream);

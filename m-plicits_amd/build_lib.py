@@ -25,11 +25,12 @@ HIP_FLAGS = [
 ]
 
 
-# Per-file additions.  level.hip, trace.hip: let the MFMAs accumulate in VGPRs (the
+# Per-file additions.  level.hip, trace.hip, batch.hip: let the MFMAs accumulate in VGPRs (the
 # epilogue then reads the accumulators without v_accvgpr_read copies).
 FILE_FLAGS = {
     "level.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "trace.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
 }
 
 

@@ -29,7 +29,7 @@
 //    overlap as far as one wave's in-order issue allows.
 #pragma once
 
-#include "stack.h"
+#include "eval_dispatch.h"
 
 namespace mp {
 
@@ -302,5 +302,72 @@ struct LevelCore {
     for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
   }
 };
+
+// Default column blocks per wave at width W: the B operand set then takes 128
+// VGPRs (W / 4 per 32-point block for 16-bit shapes, per 16-point block for
+// FP32).
+template <class S>
+constexpr int default_cols(int W) {
+  return S::ROWS == 32 ? (W == 64 ? 4 : W == 512 ? 1 : 2) : (W == 64 ? 8 : W == 128 ? 4 : W == 256 ? 2 : 1);
+}
+
+// Block `block` (4 waves x TILE points) of one level launch: evaluate, store
+// (accumulate), append in-band keys.  n: points of the launch (device count
+// for index lists), with block * 4 * TILE < n.  Every wave of the workgroup
+// calls it (the staged stream has barriers); a wave whose tile lies past n runs
+// on clamped points and stores nothing.
+template <class S, int W, int NC>
+__device__ inline void eval_block(const LevelDev& L, const EvalTarget& t, int64_t n, int64_t block,
+                                  typename S::Frag* lds) {
+  typedef LevelCore<S, W, NC, true> M;
+  const int lane = threadIdx.x & 63;
+  const int64_t base = (block * 4 + (threadIdx.x >> 6)) * M::TILE;
+
+  float x[NC][3];
+  int64_t key[NC];
+  bool valid[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int64_t i = base + c * S::ROWS + (lane % S::ROWS);
+    valid[c] = i < n;
+    const int64_t ii = valid[c] ? i : n - 1;
+    if (t.src == SRC_POINTS) {
+      key[c] = ii;
+      x[c][0] = t.xyz[3 * ii + 0];
+      x[c][1] = t.xyz[3 * ii + 1];
+      x[c][2] = t.xyz[3 * ii + 2];
+    } else {
+      key[c] = (t.src == SRC_GRID) ? t.first + ii : (int64_t)t.list[ii];
+      grid_coord(key[c], t.N, t.grid_mode, t.h, x[c]);
+    }
+  }
+  float f[NC];
+  M::eval(L, x, f, lds, lds + M::RING_FRAGS + (threadIdx.x >> 6) * M::HANDOFF_FRAGS);
+
+  // Lanes of group 0 own the store (every group holds the same value).
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    bool flag = false;
+    if (lane < S::ROWS && valid[c]) {
+      float v = f[c];
+      if (t.accumulate) v = t.out[key[c]] + v;
+      t.out[key[c]] = v;
+      flag = t.band > 0.f && fabsf(v) < t.band;
+    }
+    if (t.band > 0.f) {
+      const uint64_t m = __ballot(flag);
+      if (m) {
+        const int cnt = __popcll(m);
+        uint32_t pos = 0;
+        if (lane == 0) pos = atomicAdd(t.cand_count, (uint32_t)cnt);
+        pos = __shfl(pos, 0);
+        if (flag) {
+          const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+          t.cand[pos + __popcll(below)] = (uint32_t)key[c];
+        }
+      }
+    }
+  }
+}
 
 }  // namespace mp

@@ -58,6 +58,18 @@ class mp_trace_params(ctypes.Structure):
     ]
 
 
+class mp_extract_job(ctypes.Structure):
+    _fields_ = [
+        ("stack", ctypes.c_void_p),
+        ("N", ctypes.c_int),
+        ("grid_mode", ctypes.c_int),
+        ("chunk", ctypes.c_int64),
+        ("n_deltas", ctypes.c_int),
+        ("deltas", ctypes.c_float * (MP_MAX_LEVELS - 1)),
+        ("vol", ctypes.c_void_p),
+    ]
+
+
 # Exported symbols: name -> (restype, argtypes).  The CPU tests check that the
 # library exports every one of them.
 SIGNATURES = {
@@ -96,6 +108,11 @@ SIGNATURES = {
     "mp_train_out_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_extract_batch_plan": (ctypes.c_int, [ctypes.POINTER(mp_extract_job), ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "mp_extract_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_extract_batch_counts": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, ctypes.c_void_p]),
+    "mp_extract_batch_destroy": (None, [ctypes.c_void_p]),
     "mp_mc_workspace_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mp_mc_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_float, c_int64_p, c_int64_p, ctypes.c_void_p,
