@@ -161,6 +161,31 @@ int mp_trace(mp_stack* stack, const float* cams, int n_cams, const mp_trace_para
              float* depth, float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats,
              void* stream);
 
+/* Batched tracing of many stacks, one camera each (uneven widths, levels and
+ * image sizes; every stack in the same precision): the persistent tracer walks
+ * the jobs in order, a wave moving on to the next job once the current one has
+ * no rays left for it.  Plan / run / stats as for mp_extract_batch_*: run only
+ * launches kernels (hit counts stay on the device; the normal kernels read
+ * them), so it can be captured in a hipGraph.  stats: [n_jobs][MP_MAX_LEVELS
+ * + 1] evaluations per level, then hits (host-sync point).  No counterpart in
+ * the reference, whose renderer draws one model at a time
+ * (renderer/cuda/state.cu:816-1055). */
+typedef struct mp_trace_batch mp_trace_batch;
+typedef struct mp_trace_job {
+  mp_stack* stack;
+  const float* cam;     /* device [32]: inverse view, inverse projection (column-major) */
+  mp_trace_params params;
+  float* depth;         /* device [height][width] */
+  float* normals;       /* device [height][width][3] or NULL */
+  uint32_t* rgba;       /* device [height][width] */
+  int32_t* steps;       /* device [height][width] */
+} mp_trace_job;
+
+int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_trace_batch** out);
+int mp_trace_batch_run(mp_trace_batch* plan, void* stream);
+int mp_trace_batch_stats(const mp_trace_batch* plan, int64_t* stats, void* stream);
+void mp_trace_batch_destroy(mp_trace_batch* plan);
+
 /* Elementwise stages of the residual-level training step (value + T input
  * tangents per point, T <= 3), replacing the autograd graph of
  * loss_add_detail_fine (i3d/loss_functions.py:327-394) with its

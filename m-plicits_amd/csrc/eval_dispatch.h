@@ -29,9 +29,10 @@ int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t
 
 
 // f_level and its input gradient at n points xyz [n][3]; f may be NULL;
-// accumulate adds into f / g instead of overwriting.
+// accumulate adds into f / g instead of overwriting.  With n_dev the point
+// count is read on the device (n is then an upper bound: the launch size).
 int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n, float* f, float* g,
-                      int accumulate, hipStream_t stream);
+                      int accumulate, hipStream_t stream, const unsigned long long* n_dev = nullptr);
 
 // Input gradient of sum_{l=lvl0}^{lvl1-1} f_l at n points (overwrites grad).
 int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,

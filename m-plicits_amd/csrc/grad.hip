@@ -137,8 +137,9 @@ struct MlpGrad {
 // One point per lane column; `waves` waves per block share the dynamic LDS.
 template <class S, int W, int G>
 __global__ __launch_bounds__(256) void k_level_grad(LevelDev L, const float* __restrict__ xyz, int64_t n, float* f_out,
-                             float* g_out, int accumulate) {
+                             float* g_out, int accumulate, const unsigned long long* n_dev) {
   typedef MlpGrad<S, W, G> MG;
+  if (n_dev) n = (int64_t)*n_dev < n ? (int64_t)*n_dev : n;  // device count, n = upper bound
   extern __shared__ __align__(16) unsigned char lds[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -170,8 +171,9 @@ __global__ __launch_bounds__(256) void k_level_grad(LevelDev L, const float* __r
 // 3 -> W -> 1 (no hidden GEMM): grad = sum_u w_L[u] cos(a0[u]) W0[u].
 template <class S, int W>
 __global__ __launch_bounds__(256) void k_level_grad1(LevelDev L, const float* __restrict__ xyz, int64_t n, float* f_out,
-                              float* g_out, int accumulate) {
+                              float* g_out, int accumulate, const unsigned long long* n_dev) {
   typedef Mlp<S, W, 1> M;
+  if (n_dev) n = (int64_t)*n_dev < n ? (int64_t)*n_dev : n;
   const int lane = threadIdx.x & 63;
   const int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * S::ROWS;
   if (base >= n) return;
@@ -213,7 +215,7 @@ static constexpr size_t LDS_BUDGET = 160 * 1024;
 
 template <class S, int W, int G>
 static int launch_grad_typed(const LevelDev& L, const float* xyz, int64_t n, float* f, float* g,
-                             int acc, hipStream_t stream) {
+                             int acc, const unsigned long long* n_dev, hipStream_t stream) {
   typedef MlpGrad<S, W, G> MG;
   int waves = (int)(LDS_BUDGET / MG::LDS_PER_WAVE);
   waves = waves > 4 ? 4 : waves;
@@ -229,25 +231,25 @@ static int launch_grad_typed(const LevelDev& L, const float* xyz, int64_t n, flo
     attr_set = true;
   }
   hipLaunchKernelGGL((k_level_grad<S, W, G>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, L,
-                     xyz, n, f, g, acc);
+                     xyz, n, f, g, acc, n_dev);
   MP_LAUNCHED("k_level_grad");
   return MP_OK;
 }
 
 template <class S, int W>
 static int launch_grad_depth(const LevelDev& L, const float* xyz, int64_t n, float* f, float* g, int acc,
-                             hipStream_t stream) {
+                             const unsigned long long* n_dev, hipStream_t stream) {
   switch (L.n_hidden - 1) {
     case 0: {
       const int64_t blocks = (n + 4 * S::ROWS - 1) / (4 * S::ROWS);
       hipLaunchKernelGGL((k_level_grad1<S, W>), dim3((unsigned)blocks), dim3(256), 0, stream, L, xyz, n, f,
-                         g, acc);
+                         g, acc, n_dev);
       MP_LAUNCHED("k_level_grad1");
       return MP_OK;
     }
-    case 1: return launch_grad_typed<S, W, 1>(L, xyz, n, f, g, acc, stream);
-    case 2: return launch_grad_typed<S, W, 2>(L, xyz, n, f, g, acc, stream);
-    case 3: return launch_grad_typed<S, W, 3>(L, xyz, n, f, g, acc, stream);
+    case 1: return launch_grad_typed<S, W, 1>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 2: return launch_grad_typed<S, W, 2>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 3: return launch_grad_typed<S, W, 3>(L, xyz, n, f, g, acc, n_dev, stream);
   }
   set_error("gradient: levels with more than 4 sine layers are not supported (got %d)", L.n_hidden);
   return MP_EUNSUPPORTED;
@@ -255,24 +257,24 @@ static int launch_grad_depth(const LevelDev& L, const float* xyz, int64_t n, flo
 
 template <class S>
 static int launch_grad_width(const LevelDev& L, const float* xyz, int64_t n, float* f, float* g, int acc,
-                             hipStream_t stream) {
+                             const unsigned long long* n_dev, hipStream_t stream) {
   switch (L.width) {
-    case 64: return launch_grad_depth<S, 64>(L, xyz, n, f, g, acc, stream);
-    case 128: return launch_grad_depth<S, 128>(L, xyz, n, f, g, acc, stream);
-    case 256: return launch_grad_depth<S, 256>(L, xyz, n, f, g, acc, stream);
-    case 512: return launch_grad_depth<S, 512>(L, xyz, n, f, g, acc, stream);
+    case 64: return launch_grad_depth<S, 64>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 128: return launch_grad_depth<S, 128>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 256: return launch_grad_depth<S, 256>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 512: return launch_grad_depth<S, 512>(L, xyz, n, f, g, acc, n_dev, stream);
   }
   set_error("gradient: unsupported padded width %d", L.width);
   return MP_EUNSUPPORTED;
 }
 
 int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n, float* f, float* g,
-                      int accumulate, hipStream_t stream) {
+                      int accumulate, hipStream_t stream, const unsigned long long* n_dev) {
   const LevelDev& L = st->lv[level].dev;
   switch (st->shape) {
-    case SH_32x32x16_BF16: return launch_grad_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, stream);
-    case SH_32x32x16_F16: return launch_grad_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, stream);
-    case SH_16x16x4_F32: return launch_grad_width<S16x16x4_f32>(L, xyz, n, f, g, accumulate, stream);
+    case SH_32x32x16_BF16: return launch_grad_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, n_dev, stream);
+    case SH_32x32x16_F16: return launch_grad_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, n_dev, stream);
+    case SH_16x16x4_F32: return launch_grad_width<S16x16x4_f32>(L, xyz, n, f, g, accumulate, n_dev, stream);
   }
   set_error("gradient: unsupported fragment shape %d", st->shape);
   return MP_EUNSUPPORTED;

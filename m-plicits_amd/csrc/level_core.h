@@ -76,7 +76,7 @@ struct LevelCore {
 
     // Stage `stage` into ring slot `slot`: each of the four waves copies 4 of
     // the stage's 16 fragments (one 16-byte direct load per lane each).
-    __device__ inline void copy_stage(int64_t stage, int slot) {
+    __device__ __forceinline__ void copy_stage(int64_t stage, int slot) {
 #pragma unroll
       for (int i = 0; i < SF / 4; ++i) {
         const int f = (SF / 4) * wave + i;
@@ -84,11 +84,11 @@ struct LevelCore {
                                          (lds_void*)(ring + (slot * SF + f) * 64), 16, 0, 0);
       }
     }
-    __device__ inline void barrier() {
+    __device__ __forceinline__ void barrier() {
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
     }
-    __device__ inline void start() {
+    __device__ __forceinline__ void start() {
       if constexpr (STAGED) {
         copy_stage(0, 0);
         copy_stage(1, 1);
@@ -97,7 +97,7 @@ struct LevelCore {
         for (int d = 0; d < PF; ++d) pf[d] = A[d * 64 + lane];
       }
     }
-    __device__ inline void begin() {
+    __device__ __forceinline__ void begin() {
       if constexpr (STAGED) {
         barrier();
 #pragma unroll
@@ -105,7 +105,7 @@ struct LevelCore {
       }
     }
     // Fragment of step s of hidden layer j; issues the look-ahead.
-    __device__ inline Frag next(int j, int s) {
+    __device__ __forceinline__ Frag next(int j, int s) {
       if constexpr (STAGED) {
         const int t = s + D;  // step whose fragment is read now
         if (t % SF == 0) {
@@ -124,12 +124,12 @@ struct LevelCore {
     }
     // Outstanding look-ahead copies land in the ring; let them finish before
     // the workgroup can retire and its LDS be reassigned.
-    __device__ inline void finish() {
+    __device__ __forceinline__ void finish() {
       if constexpr (STAGED) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
   };
 
-  static __device__ inline void loadc(const float* __restrict__ p, Acc& v) {
+  static __device__ __forceinline__ void loadc(const float* __restrict__ p, Acc& v) {
     if constexpr (CREGS % 4 == 0) {
       const float4* p4 = reinterpret_cast<const float4*>(p);
 #pragma unroll
@@ -143,13 +143,13 @@ struct LevelCore {
     }
   }
 
-  static __device__ inline float group_sum(float v) {
+  static __device__ __forceinline__ float group_sum(float v) {
     if (GROUPS == 4) v += __shfl_xor(v, 16);
     return v + __shfl_xor(v, 32);
   }
 
   // Layer 0 into the B registers of layer 1.
-  static __device__ inline void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][KBS],
+  static __device__ __forceinline__ void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][KBS],
                                       int lane, int g) {
     if constexpr (MFMA_FIRST) {
       float bxy[NC], bz1[NC];
@@ -195,7 +195,7 @@ struct LevelCore {
   }
 
   // A level with one sine layer: f = w_L . sin(W_0 x + b_0) on the VALU.
-  static __device__ inline void single(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC], int g) {
+  static __device__ __forceinline__ void single(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC], int g) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = 0.f;
     for (int v0 = 0; v0 < W; v0 += KB) {
@@ -217,7 +217,7 @@ struct LevelCore {
   // in slices between the MFMAs of block rb.  epi(rb, q, acc) retires
   // accumulator register q of row block rb.
   template <class Epi>
-  static __device__ inline void gemm(Stream& A, int j, const float* __restrict__ bc,
+  static __device__ __forceinline__ void gemm(Stream& A, int j, const float* __restrict__ bc,
                                      const Frag (&b)[NC][KBS], int g, Epi epi) {
     Acc acc[2][NC];
     Acc bias;
@@ -248,7 +248,7 @@ struct LevelCore {
   // same value for its column).  ring: the workgroup's staging ring (staged
   // stream only; then every wave of the workgroup must call it); hand: this
   // wave's handoff region of HANDOFF_FRAGS fragments.
-  static __device__ inline void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
+  static __device__ __forceinline__ void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
                                      Frag* ring, Frag* hand) {
     const int lane = threadIdx.x & 63;
     const int g = lane / ROWS;
@@ -317,7 +317,7 @@ constexpr int default_cols(int W) {
 // calls it (the staged stream has barriers); a wave whose tile lies past n runs
 // on clamped points and stores nothing.
 template <class S, int W, int NC>
-__device__ inline void eval_block(const LevelDev& L, const EvalTarget& t, int64_t n, int64_t block,
+__device__ __forceinline__ void eval_block(const LevelDev& L, const EvalTarget& t, int64_t n, int64_t block,
                                   typename S::Frag* lds) {
   typedef LevelCore<S, W, NC, true> M;
   const int lane = threadIdx.x & 63;

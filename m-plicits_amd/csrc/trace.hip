@@ -23,6 +23,19 @@
 #include "level_core.h"
 #include "trace.h"
 
+#include <vector>
+
+struct mp_trace_batch {
+  int shape = 0, n_jobs = 0, nc = 1, blocks = 0;
+  void* mem = nullptr;             // tables, counters, per-job scratch (one allocation)
+  void* jobs = nullptr;            // TraceArgs [n_jobs] (device)
+  void* outs = nullptr;            // TraceOut [n_jobs] (device)
+  unsigned long long* ctr = nullptr;  // [n_jobs][8]: next ray, evaluations per level, hits
+  std::vector<mp_stack*> stacks;
+  std::vector<int64_t> rays;
+  std::vector<float*> cpos, grad;
+};
+
 namespace mp {
 
 __global__ void k_fill_ull(unsigned long long* p, int n, unsigned long long v) {
@@ -36,7 +49,7 @@ constexpr size_t TRACE_HAND_BYTES = 32 * 1024;
 
 // f_level at the NC points of this lane; hand: the wave's handoff region.
 template <class S, int NC>
-__device__ inline void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
+__device__ __forceinline__ void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
                                       typename S::Frag* hand) {
   switch (L.width) {
     case 64: LevelCore<S, 64, NC, false>::eval(L, x, f, nullptr, hand); break;
@@ -88,11 +101,10 @@ __device__ inline void ray_init(const TraceArgs& a, int64_t rid, float o[3], flo
   }
 }
 
+// Trace the rays of one job until its queue is empty and this wave's slots
+// have all finished.  hand: the wave's LDS handoff region.
 template <class S, int NC>
-__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
-  extern __shared__ uint4 lds_raw[];
-  typename S::Frag* hand = reinterpret_cast<typename S::Frag*>(
-      reinterpret_cast<char*>(lds_raw) + (threadIdx.x >> 6) * TRACE_HAND_BYTES);
+__device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag* hand) {
   const int lane = threadIdx.x & 63;
   const int col = lane % S::ROWS;
   const int64_t total = a.total_rays;
@@ -237,6 +249,24 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
       if (evals[l]) atomicAdd(&a.stats[l], evals[l]);
 }
 
+template <class S, int NC>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+  extern __shared__ uint4 lds_raw[];
+  trace_rays<S, NC>(a, reinterpret_cast<typename S::Frag*>(reinterpret_cast<char*>(lds_raw) +
+                                                          (threadIdx.x >> 6) * TRACE_HAND_BYTES));
+}
+
+// Every job in turn: a wave moves to the next job once the current one has no
+// rays left for it, so tiles never mix networks and idle waves help with the
+// jobs still running.
+template <class S, int NC>
+__global__ __launch_bounds__(256) void k_trace_jobs(const TraceArgs* __restrict__ jobs, int n_jobs) {
+  extern __shared__ uint4 lds_raw[];
+  typename S::Frag* hand = reinterpret_cast<typename S::Frag*>(reinterpret_cast<char*>(lds_raw) +
+                                                               (threadIdx.x >> 6) * TRACE_HAND_BYTES);
+  for (int j = 0; j < n_jobs; ++j) trace_rays<S, NC>(jobs[j], hand);
+}
+
 // Compact the hit rays: ids and positions.
 __global__ void k_collect_hits(const float* depth, const float* hit_pos, int64_t n, uint32_t* ids,
                                float* pos, unsigned long long* count) {
@@ -275,9 +305,52 @@ __global__ void k_background(uint32_t* rgba, float* normals, int64_t n) {
   }
 }
 
-__global__ void k_shade(TraceArgs a, const uint32_t* ids, const float* grad, int64_t nh) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nh) return;
+
+// Per-job outputs of the batched tracer beside TraceArgs.
+struct TraceOut {
+  uint32_t* ids;              // [rays] hit ray ids, compacted
+  float* cpos;                // [rays][3] hit positions, compacted
+  float* grad;                // [rays][3] gradient at the hits
+  unsigned long long* nhits;  // device count
+};
+
+// Background for every ray of every job, then the hits compacted per job.
+__global__ void k_collect_jobs(const TraceArgs* __restrict__ jobs, const TraceOut* __restrict__ outs, int n_jobs) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  for (int j = 0; j < n_jobs; ++j) {
+    const TraceArgs& a = jobs[j];
+    const TraceOut& o = outs[j];
+    const int64_t n = a.total_rays;
+    // wave-uniform loop bound: every lane of a wave takes the ballots
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); w < n; w += stride) {
+      const int64_t i = w + lane;
+      bool hit = false;
+      if (i < n) {
+        a.rgba[i] = rgba8(1.f, 1.f, 1.f);
+        if (a.normals) {
+          a.normals[3 * i] = 0.f; a.normals[3 * i + 1] = 0.f; a.normals[3 * i + 2] = 0.f;
+        }
+        hit = a.depth[i] >= 0.f;
+      }
+      const uint64_t m = __ballot(hit);
+      if (!m) continue;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(o.nhits, (unsigned long long)__popcll(m));
+      base = __shfl(base, 0);
+      if (hit) {
+        const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+        const unsigned long long k = base + __popcll(below);
+        o.ids[k] = (uint32_t)i;
+        o.cpos[3 * k + 0] = a.hit_pos[3 * i + 0];
+        o.cpos[3 * k + 1] = a.hit_pos[3 * i + 1];
+        o.cpos[3 * k + 2] = a.hit_pos[3 * i + 2];
+      }
+    }
+  }
+}
+
+__device__ inline void shade_one(const TraceArgs& a, const uint32_t* ids, const float* grad, int64_t k) {
   const int64_t id = ids[k];
   float o[3], d[3], t0;
   bool inside;
@@ -292,6 +365,33 @@ __global__ void k_shade(TraceArgs a, const uint32_t* ids, const float* grad, int
   if (a.normals) {
     a.normals[3 * id] = nx; a.normals[3 * id + 1] = ny; a.normals[3 * id + 2] = nz;
   }
+}
+
+__global__ void k_shade(TraceArgs a, const uint32_t* ids, const float* grad, int64_t nh) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nh) shade_one(a, ids, grad, k);
+}
+
+__global__ void k_shade_jobs(const TraceArgs* __restrict__ jobs, const TraceOut* __restrict__ outs, int n_jobs) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int j = 0; j < n_jobs; ++j) {
+    const int64_t nh = (int64_t)*outs[j].nhits;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += stride)
+      shade_one(jobs[j], outs[j].ids, outs[j].grad, k);
+  }
+}
+
+template <class S, int NC>
+static int launch_trace_jobs(int blocks, size_t lds, hipStream_t s, const TraceArgs* jobs, int n_jobs) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trace_jobs<S, NC>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_trace_jobs<S, NC>), dim3(blocks), dim3(256), lds, s, jobs, n_jobs);
+  MP_LAUNCHED("k_trace_jobs");
+  return MP_OK;
 }
 
 template <class S, int NC>
@@ -410,6 +510,167 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   if (stats) {
     for (int l = 0; l < st->n_levels; ++l) stats[l] = (int64_t)h[1 + l];
     stats[MP_MAX_LEVELS] = nh;
+  }
+  return MP_OK;
+}
+
+static constexpr int TB_CTR = 8;  // counters per job: next ray, evaluations [4], hits at [7]
+
+void mp_trace_batch_destroy(mp_trace_batch* p) {
+  if (!p) return;
+  if (p->mem) (void)hipFree(p->mem);
+  delete p;
+}
+
+int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_trace_batch** out) {
+  MP_REQUIRE(out != nullptr, MP_EINVAL, "mp_trace_batch_plan: out is NULL");
+  *out = nullptr;
+  MP_REQUIRE(jobs != nullptr && n_jobs >= 1, MP_EINVAL, "mp_trace_batch_plan: no jobs");
+  hipStream_t s = as_stream(stream);
+  const int shape = jobs[0].stack ? jobs[0].stack->shape : -1;
+  int maxw = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    const mp_trace_job& J = jobs[j];
+    const mp_trace_params* prm = &J.params;
+    MP_REQUIRE(J.stack != nullptr && J.cam != nullptr, MP_EINVAL, "mp_trace_batch_plan: job %d: NULL stack or camera", j);
+    MP_REQUIRE(J.stack->shape == shape, MP_EINVAL, "mp_trace_batch_plan: job %d: every stack must have the same precision", j);
+    MP_REQUIRE(J.depth != nullptr && J.steps != nullptr && J.rgba != nullptr, MP_EINVAL,
+               "mp_trace_batch_plan: job %d: depth, steps and rgba are required", j);
+    MP_REQUIRE(prm->width >= 1 && prm->height >= 1 && prm->max_steps >= 1 && prm->hit_eps > 0.f &&
+                   prm->bound_radius > 0.f, MP_EINVAL, "mp_trace_batch_plan: job %d: bad parameters", j);
+    MP_REQUIRE(prm->n_deltas == J.stack->n_levels - 1, MP_EINVAL, "mp_trace_batch_plan: job %d: need %d deltas, got %d",
+               j, J.stack->n_levels - 1, prm->n_deltas);
+    MP_REQUIRE((int64_t)prm->width * prm->height < (1ll << 32), MP_EUNSUPPORTED, "mp_trace_batch_plan: job %d: too many rays", j);
+    for (int l = 0; l < J.stack->n_levels; ++l) maxw = J.stack->lv[l].width > maxw ? J.stack->lv[l].width : maxw;
+  }
+  mp_trace_batch* p = new mp_trace_batch();
+  p->shape = shape;
+  p->n_jobs = n_jobs;
+  p->nc = (jobs[0].stack->nc == 2 && maxw <= 256) ? 2 : 1;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t off_jobs = off; off += al(sizeof(TraceArgs) * n_jobs);
+  const size_t off_outs = off; off += al(sizeof(TraceOut) * n_jobs);
+  const size_t off_ctr = off; off += al(sizeof(unsigned long long) * TB_CTR * n_jobs);
+  std::vector<size_t> o_pos(n_jobs), o_ids(n_jobs), o_cpos(n_jobs), o_grad(n_jobs);
+  for (int j = 0; j < n_jobs; ++j) {
+    const int64_t n = (int64_t)jobs[j].params.width * jobs[j].params.height;
+    o_pos[j] = off; off += al((size_t)n * 12);
+    o_ids[j] = off; off += al((size_t)n * 4);
+    o_cpos[j] = off; off += al((size_t)n * 12);
+    o_grad[j] = off; off += al((size_t)n * 12);
+  }
+  hipError_t e = hipMalloc(&p->mem, off);
+  if (e != hipSuccess) {
+    delete p;
+    set_error("mp_trace_batch_plan: hipMalloc(%zu) failed: %s", off, hipGetErrorString(e));
+    return MP_ENOMEM;
+  }
+  char* base = reinterpret_cast<char*>(p->mem);
+  p->jobs = base + off_jobs;
+  p->outs = base + off_outs;
+  p->ctr = reinterpret_cast<unsigned long long*>(base + off_ctr);
+  std::vector<TraceArgs> ta(n_jobs);
+  std::vector<TraceOut> to(n_jobs);
+  for (int j = 0; j < n_jobs; ++j) {
+    const mp_trace_job& J = jobs[j];
+    TraceArgs& a = ta[j];
+    a = TraceArgs{};
+    a.cams = J.cam;
+    a.width = J.params.width;
+    a.height = J.params.height;
+    a.total_rays = (int64_t)a.width * a.height;
+    a.max_steps = J.params.max_steps;
+    a.hit_eps = J.params.hit_eps;
+    a.bound_radius = J.params.bound_radius;
+    a.band_scale = J.params.band_scale;
+    a.pixel_center = J.params.pixel_center;
+    a.flip_y = J.params.flip_y;
+    a.n_levels = J.stack->n_levels;
+    for (int l = 0; l < a.n_levels; ++l) a.levels[l] = J.stack->lv[l].dev;
+    for (int i = 0; i < J.params.n_deltas; ++i) a.deltas[i] = J.params.deltas[i];
+    unsigned long long* c = p->ctr + (size_t)j * TB_CTR;
+    a.next_ray = &c[0];
+    a.stats = &c[1];
+    a.depth = J.depth;
+    a.steps = J.steps;
+    a.hit_pos = reinterpret_cast<float*>(base + o_pos[j]);
+    a.rgba = J.rgba;
+    a.normals = J.normals;
+    to[j].ids = reinterpret_cast<uint32_t*>(base + o_ids[j]);
+    to[j].cpos = reinterpret_cast<float*>(base + o_cpos[j]);
+    to[j].grad = reinterpret_cast<float*>(base + o_grad[j]);
+    to[j].nhits = &c[TB_CTR - 1];
+    p->stacks.push_back(J.stack);
+    p->rays.push_back(a.total_rays);
+    p->cpos.push_back(to[j].cpos);
+    p->grad.push_back(to[j].grad);
+  }
+  e = hipMemcpyAsync(p->jobs, ta.data(), sizeof(TraceArgs) * n_jobs, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(p->outs, to.data(), sizeof(TraceOut) * n_jobs, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the host tables go out of scope
+  int dev = 0, cus = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) {
+    mp_trace_batch_destroy(p);
+    set_error("mp_trace_batch_plan: %s", hipGetErrorString(e));
+    return MP_EHIP;
+  }
+  p->blocks = cus;
+  *out = p;
+  return MP_OK;
+}
+
+int mp_trace_batch_run(mp_trace_batch* p, void* stream) {
+  MP_REQUIRE(p != nullptr, MP_EINVAL, "mp_trace_batch_run: plan is NULL");
+  hipStream_t s = as_stream(stream);
+  const int n_ctr = TB_CTR * p->n_jobs;
+  hipLaunchKernelGGL(k_fill_ull, dim3((n_ctr + 255) / 256), dim3(256), 0, s, p->ctr, n_ctr, 0ull);
+  MP_LAUNCHED("k_fill_ull");
+  const TraceArgs* jobs = reinterpret_cast<const TraceArgs*>(p->jobs);
+  const TraceOut* outs = reinterpret_cast<const TraceOut*>(p->outs);
+  const size_t lds = 4 * TRACE_HAND_BYTES;
+  int rc = MP_OK;
+  switch (p->shape) {
+    case SH_32x32x16_BF16:
+      rc = p->nc == 2 ? launch_trace_jobs<S32x32x16_bf16, 2>(p->blocks, lds, s, jobs, p->n_jobs)
+                      : launch_trace_jobs<S32x32x16_bf16, 1>(p->blocks, lds, s, jobs, p->n_jobs);
+      break;
+    case SH_32x32x16_F16:
+      rc = p->nc == 2 ? launch_trace_jobs<S32x32x16_f16, 2>(p->blocks, lds, s, jobs, p->n_jobs)
+                      : launch_trace_jobs<S32x32x16_f16, 1>(p->blocks, lds, s, jobs, p->n_jobs);
+      break;
+    default:
+      rc = launch_trace_jobs<S16x16x4_f32, 1>(p->blocks, lds, s, jobs, p->n_jobs);
+      break;
+  }
+  if (rc != MP_OK) return rc;
+  hipLaunchKernelGGL(k_collect_jobs, dim3(p->blocks * 4), dim3(256), 0, s, jobs, outs, p->n_jobs);
+  MP_LAUNCHED("k_collect_jobs");
+  // analytic normals at the hits: the hit counts stay on the device
+  for (int j = 0; j < p->n_jobs; ++j) {
+    const unsigned long long* nh = p->ctr + (size_t)j * TB_CTR + TB_CTR - 1;
+    for (int l = 0; l < p->stacks[j]->n_levels; ++l) {
+      rc = launch_level_grad(p->stacks[j], l, p->cpos[j], p->rays[j], nullptr, p->grad[j], l > 0, s, nh);
+      if (rc != MP_OK) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_shade_jobs, dim3(p->blocks * 4), dim3(256), 0, s, jobs, outs, p->n_jobs);
+  MP_LAUNCHED("k_shade_jobs");
+  return MP_OK;
+}
+
+int mp_trace_batch_stats(const mp_trace_batch* p, int64_t* stats, void* stream) {
+  MP_REQUIRE(p != nullptr && stats != nullptr, MP_EINVAL, "mp_trace_batch_stats: NULL argument");
+  hipStream_t s = as_stream(stream);
+  std::vector<unsigned long long> h((size_t)TB_CTR * p->n_jobs);
+  MP_HIP(hipMemcpyAsync(h.data(), p->ctr, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  MP_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < p->n_jobs; ++j) {
+    for (int l = 0; l < MP_MAX_LEVELS; ++l)
+      stats[(size_t)j * (MP_MAX_LEVELS + 1) + l] = (int64_t)h[(size_t)j * TB_CTR + 1 + l];
+    stats[(size_t)j * (MP_MAX_LEVELS + 1) + MP_MAX_LEVELS] = (int64_t)h[(size_t)j * TB_CTR + TB_CTR - 1];
   }
   return MP_OK;
 }

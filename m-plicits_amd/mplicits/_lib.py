@@ -70,6 +70,18 @@ class mp_extract_job(ctypes.Structure):
     ]
 
 
+class mp_trace_job(ctypes.Structure):
+    _fields_ = [
+        ("stack", ctypes.c_void_p),
+        ("cam", ctypes.c_void_p),
+        ("params", mp_trace_params),
+        ("depth", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("rgba", ctypes.c_void_p),
+        ("steps", ctypes.c_void_p),
+    ]
+
+
 # Exported symbols: name -> (restype, argtypes).  The CPU tests check that the
 # library exports every one of them.
 SIGNATURES = {
@@ -91,6 +103,11 @@ SIGNATURES = {
     "mp_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                 c_int64_p, ctypes.c_void_p]),
+    "mp_trace_batch_plan": (ctypes.c_int, [ctypes.POINTER(mp_trace_job), ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "mp_trace_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_trace_batch_stats": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, ctypes.c_void_p]),
+    "mp_trace_batch_destroy": (None, [ctypes.c_void_p]),
     "mp_train_first": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),

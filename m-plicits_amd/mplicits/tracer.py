@@ -26,6 +26,16 @@ def camera_block(cameras, device="cuda"):
     return torch.tensor(np.stack(rows), dtype=torch.float32, device=device)
 
 
+def _params(width, height, deltas, max_steps, hit_eps, bound_radius, band_scale, pixel_center, flip_y):
+    prm = _lib.mp_trace_params()
+    prm.width, prm.height, prm.max_steps = int(width), int(height), int(max_steps)
+    prm.hit_eps, prm.bound_radius, prm.band_scale = float(hit_eps), float(bound_radius), float(band_scale)
+    prm.pixel_center, prm.flip_y, prm.n_deltas = int(bool(pixel_center)), int(bool(flip_y)), len(deltas)
+    for i, d in enumerate(deltas):
+        prm.deltas[i] = float(d)
+    return prm
+
+
 def trace(stack, cameras, width, height, deltas=(), max_steps=128, hit_eps=1e-4, bound_radius=1.2,
           band_scale=1.4, pixel_center=True, flip_y=False, normals=True):
     """Trace every pixel of every camera through ``stack`` (a LevelStack or a
@@ -47,12 +57,7 @@ def trace(stack, cameras, width, height, deltas=(), max_steps=128, hit_eps=1e-4,
     deltas = [float(d) for d in deltas]
     if len(deltas) != stack.n_levels - 1:
         raise ValueError(f"need {stack.n_levels - 1} deltas, got {len(deltas)}")
-    prm = _lib.mp_trace_params()
-    prm.width, prm.height, prm.max_steps = int(width), int(height), int(max_steps)
-    prm.hit_eps, prm.bound_radius, prm.band_scale = float(hit_eps), float(bound_radius), float(band_scale)
-    prm.pixel_center, prm.flip_y, prm.n_deltas = int(bool(pixel_center)), int(bool(flip_y)), len(deltas)
-    for i, d in enumerate(deltas):
-        prm.deltas[i] = d
+    prm = _params(width, height, deltas, max_steps, hit_eps, bound_radius, band_scale, pixel_center, flip_y)
     dev = stack.device
     depth = torch.empty(n, height, width, dtype=torch.float32, device=dev)
     steps = torch.empty(n, height, width, dtype=torch.int32, device=dev)
@@ -68,3 +73,81 @@ def trace(stack, cameras, width, height, deltas=(), max_steps=128, hit_eps=1e-4,
     if normals:
         out["normals"] = nrm
     return out
+
+
+class BatchedTrace:
+    """Many stacks, one camera each, traced by one persistent launch
+    (mp_trace_batch_*); the run is captured once in a CUDA/HIP graph and
+    replayed.
+
+    jobs: sequence of (stack, camera, width, height, deltas) with ``stack`` a
+    LevelStack (all in one precision) and ``camera`` an (inv_view, inv_proj)
+    pair or a [32] float32 tensor.  Outputs per job: ``depth[j]`` [H, W],
+    ``steps[j]``, ``rgba[j]`` and ``normals[j]`` [H, W, 3].
+    """
+
+    def __init__(self, jobs, max_steps=128, hit_eps=1e-4, bound_radius=1.2, band_scale=1.4,
+                 pixel_center=True, flip_y=False, normals=True):
+        lib = _lib.load()
+        self._lib = lib
+        self.stacks, self.cams = [], []
+        self.depth, self.steps, self.rgba, self.normals = [], [], [], []
+        cj = (_lib.mp_trace_job * len(jobs))()
+        for j, (stack, cam, width, height, deltas) in enumerate(jobs):
+            stack.refresh()
+            dev = stack.device
+            c = cam if isinstance(cam, torch.Tensor) else camera_block([cam], dev)[0]
+            c = c.to(dev, torch.float32).reshape(32).contiguous()
+            self.stacks.append(stack)
+            self.cams.append(c)
+            self.depth.append(torch.empty(height, width, dtype=torch.float32, device=dev))
+            self.steps.append(torch.empty(height, width, dtype=torch.int32, device=dev))
+            self.rgba.append(torch.empty(height, width, dtype=torch.int32, device=dev))
+            self.normals.append(torch.empty(height, width, 3, dtype=torch.float32, device=dev) if normals else None)
+            cj[j].stack = stack._handle.value
+            cj[j].cam = c.data_ptr()
+            cj[j].params = _params(width, height, list(deltas), max_steps, hit_eps, bound_radius, band_scale,
+                                   pixel_center, flip_y)
+            cj[j].depth = self.depth[-1].data_ptr()
+            cj[j].normals = self.normals[-1].data_ptr() if normals else None
+            cj[j].rgba = self.rgba[-1].data_ptr()
+            cj[j].steps = self.steps[-1].data_ptr()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.mp_trace_batch_plan(cj, len(jobs), _lib.stream_handle(), ctypes.byref(handle)))
+        self._handle = handle
+        self.graph = None
+
+    def run(self, graph=True):
+        if not graph:
+            _lib.check(self._lib.mp_trace_batch_run(self._handle, _lib.stream_handle()))
+            return
+        if self.graph is None:
+            self.run(graph=False)  # kernel attributes and code objects are set up eagerly
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                _lib.check(self._lib.mp_trace_batch_run(self._handle, _lib.stream_handle()))
+            self.graph = g
+        self.graph.replay()
+
+    def stats(self):
+        """Per job: evaluations per level and hits of the last run (synchronises)."""
+        n = len(self.stacks)
+        k = _lib.MP_MAX_LEVELS + 1
+        buf = (ctypes.c_int64 * (n * k))()
+        _lib.check(self._lib.mp_trace_batch_stats(self._handle, ctypes.cast(buf, _lib.c_int64_p),
+                                                   _lib.stream_handle()))
+        return [{"evals": [buf[j * k + l] for l in range(st.n_levels)], "hits": buf[j * k + k - 1]}
+                for j, st in enumerate(self.stacks)]
+
+    def close(self):
+        self.graph = None
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            self._lib.mp_trace_batch_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -64,3 +64,31 @@ def test_batch_rejects_mixed_precision():
     c = LevelStack(_model(64, 1, 2), "fp32")
     with pytest.raises(ValueError):
         BatchedExtraction([a, c], [16, 16], [[], []])
+
+
+def test_trace_batch_matches_single_model_trace():
+    from mplicits.synthetic import orbit_cameras
+    from mplicits.tracer import BatchedTrace, trace
+    specs = [(64, 1, (40, 24), []), (256, 3, (33, 20), [0.05, 0.02]), (512, 2, (48, 27), [0.04]),
+             (128, 2, (20, 31), [0.05])]
+    stacks = [LevelStack(_model(w, l, 300 + k), "bf16") for k, (w, l, _, _) in enumerate(specs)]
+    cams = orbit_cameras(len(specs), seed=5, aspect=1.5)
+    refs = []
+    for st, cam, (_, _, (W, H), d) in zip(stacks, cams, specs):
+        out = trace(st, [cam], W, H, deltas=d, max_steps=64)
+        refs.append({k: (v[0].clone() if isinstance(v, torch.Tensor) else v) for k, v in out.items()})
+    b = BatchedTrace([(st, cam, W, H, d) for st, cam, (_, _, (W, H), d) in zip(stacks, cams, specs)],
+                     max_steps=64)
+    for rep in range(3):
+        for t in b.depth + b.rgba:
+            t.fill_(7)
+        b.run(graph=rep > 0)
+        torch.cuda.synchronize()
+        st = b.stats()
+        for j, ref in enumerate(refs):
+            assert torch.equal(b.depth[j], ref["depth"]), f"run {rep}, job {j}"
+            assert torch.equal(b.steps[j], ref["steps"]), f"run {rep}, job {j}"
+            assert torch.equal(b.rgba[j], ref["rgba"]), f"run {rep}, job {j}"
+            assert torch.equal(b.normals[j], ref["normals"]), f"run {rep}, job {j}"
+            assert st[j]["hits"] == ref["hits"] and st[j]["evals"] == ref["evals"]
+    b.close()
