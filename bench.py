@@ -33,8 +33,9 @@ sys.path.insert(0, osp.join(ROOT, "m-plicits_amd"))
 from mplicits.meshing import marching_cubes_gpu  # noqa: E402
 from mplicits.model import from_pth  # noqa: E402
 from mplicits.stack import LevelStack  # noqa: E402
-from mplicits.synthetic import DELTAS, MASTER_SEED, BumpySphere, orbit_cameras, residual_levels  # noqa: E402
-from mplicits.tracer import camera_block, trace  # noqa: E402
+from mplicits.batch import BatchedExtraction  # noqa: E402
+from mplicits.synthetic import DELTAS, MASTER_SEED, BumpySphere, orbit_cameras, residual_levels, uneven_workload  # noqa: E402
+from mplicits.tracer import BatchedTrace, camera_block, trace  # noqa: E402
 from mplicits import training  # noqa: E402
 
 N = 512
@@ -149,22 +150,76 @@ def cpu_baseline(levels, budget_chunks=96):
 
 
 def bench_trace(levels, precisions, reps=2):
-    """16 seeded orbit cameras at 1920x1080 through the coarse model (one
-    persistent launch + normals + shading per frame set): rays per second."""
+    """16 seeded orbit cameras at 1920x1080 (one persistent launch + normals +
+    shading per frame set), through the coarse model alone and through the
+    three-level stack (bands 0.05, 0.02): rays per second."""
     cams = camera_block(orbit_cameras(16, seed=MASTER_SEED + 7))
     out = {}
-    for prec in precisions:
-        st = LevelStack([levels[0]], prec)
-        r = trace(st, cams, 1920, 1080, [])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            r = trace(st, cams, 1920, 1080, [])
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
-        out[prec] = {"rays_per_s": round(16 * 1920 * 1080 / dt, 1), "ms": round(dt * 1e3, 2), "hits": r["hits"],
-                     "coarse_evals": r["evals"][0]}
+    for name, lv, deltas in (("coarse", levels[:1], []), ("stack3", levels, list(DELTAS))):
+        for prec in precisions:
+            st = LevelStack(lv, prec)
+            r = trace(st, cams, 1920, 1080, deltas)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r = trace(st, cams, 1920, 1080, deltas)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            key = prec if name == "coarse" else f"{name}_{prec}"
+            out[key] = {"rays_per_s": round(16 * 1920 * 1080 / dt, 1), "ms": round(dt * 1e3, 2),
+                        "hits": r["hits"], "evals": r["evals"]}
     return out
+
+
+def bench_batch(precision="bf16", reps=3, n_models=24):
+    """Batched uneven workload: 24 seeded models (width 64-512, 1-3 levels,
+    own bumpy-sphere target, grids 128^3-512^3, one 1280x720 camera each).
+    Extraction and tracing each run as one captured graph of persistent
+    launches (timed over graph replays); marching cubes per model, eagerly."""
+    work = uneven_workload(n_models, device="cuda")
+    stacks = [LevelStack(m["levels"], precision) for m in work]
+    ext = BatchedExtraction(stacks, [m["N"] for m in work], [m["deltas"] for m in work], chunk=CHUNK)
+    ext.run(graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ext.run(graph=True)
+    torch.cuda.synchronize()
+    ext_ms = (time.perf_counter() - t0) / reps * 1e3
+    counts = ext.counts()
+    flop = sum(c * level_flops(lv.hidden_widths()) for m, cs in zip(work, counts) for c, lv in zip(cs, m["levels"]))
+    t0 = time.perf_counter()
+    faces = 0
+    for j, m in enumerate(work):
+        _, f, _, _ = marching_cubes_gpu(ext.volume(j), 0.0)
+        faces += int(f.shape[0])
+    torch.cuda.synchronize()
+    mc_ms = (time.perf_counter() - t0) * 1e3
+    ext.close()
+    del ext
+    tr = BatchedTrace([(st, m["camera"], 1280, 720, m["deltas"]) for st, m in zip(stacks, work)])
+    tr.run(graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tr.run(graph=True)
+    torch.cuda.synchronize()
+    tr_ms = (time.perf_counter() - t0) / reps * 1e3
+    stats = tr.stats()
+    tr.close()
+    rays = n_models * 1280 * 720
+    widths = {}
+    for m in work:
+        widths[m["width"]] = widths.get(m["width"], 0) + 1
+    return {"precision": precision, "models": n_models, "models_per_width": widths,
+            "levels": [len(m["levels"]) for m in work], "grids": [m["N"] for m in work],
+            "grid_points": int(sum(m["N"] ** 3 for m in work)),
+            "samples_per_level": [int(sum(c[l] for c in counts if l < len(c))) for l in range(3)],
+            "extract_ms": round(ext_ms, 2), "extract_tflops": round(flop / (ext_ms * 1e-3) / 1e12, 1),
+            "mc_ms": round(mc_ms, 2), "faces": faces,
+            "trace_ms": round(tr_ms, 2), "rays": rays, "rays_per_s": round(rays / (tr_ms * 1e-3), 1),
+            "hits": int(sum(s["hits"] for s in stats)),
+            "graphs": "extraction and tracing each captured once, timed over replays"}
 
 
 def bench_train(levels, precisions, steps=4):
@@ -240,10 +295,14 @@ def main():
         del st
 
     tr = bench_trace(levels, ["bf16", "fp32"]) if not args.no_extras else None
-    tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
     if tr:
         print(f"# trace: {tr}", file=sys.stderr, flush=True)
+    tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
+    if tn:
         print(f"# train: {tn}", file=sys.stderr, flush=True)
+    bt = bench_batch() if not args.no_extras else None
+    if bt:
+        print(f"# batch: {bt}", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel (coarse level over the whole grid)
     coarse_flop = level_flops(levels[0].hidden_widths()) * N ** 3
@@ -280,6 +339,7 @@ def main():
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
         "trace_16x1080p": tr,
         "train_step_2e18": tn,
+        "batch_24_uneven": bt,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

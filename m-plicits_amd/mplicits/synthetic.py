@@ -163,3 +163,54 @@ def orbit_cameras(n, seed, radius=2.5, fovy=45.0, aspect=16 / 9, elev_deg=30.0):
         eye = radius * np.array([math.cos(e) * math.sin(a), math.sin(e), math.cos(e) * math.cos(a)])
         cams.append((np.linalg.inv(look_at(eye)), np.linalg.inv(perspective(fovy, aspect))))
     return cams
+
+
+def _hidden_features(model, x):
+    """Last sine layer's activations of a baked SIREN (float64 torch ops; data
+    construction only)."""
+    h = x.to(torch.float64)
+    for seq in model.net[:-1]:
+        lin = seq[0]
+        h = torch.sin(float(seq[1].w0) * (h @ lin.weight.detach().to(torch.float64).T + lin.bias.detach().to(torch.float64)))
+    return h
+
+
+def fit_output_layer(model, shape, n=20000, seed=0, ridge=1e-6):
+    """Solve the output layer of ``model`` by ridge least squares so that the
+    network approximates ``shape.sdf`` (random-feature regression on the
+    config-1 point mix).  Cheap stand-in for a full fit when many models need
+    a surface."""
+    dev = model.net[0][0].weight.device
+    x, y, _ = training_batch(shape, n, seed, device=dev)
+    h = _hidden_features(model, x)
+    a = torch.cat([h, torch.ones(h.shape[0], 1, dtype=h.dtype, device=h.device)], 1)
+    lhs = a.T @ a + ridge * a.shape[0] * torch.eye(a.shape[1], dtype=a.dtype, device=a.device)
+    sol = torch.linalg.solve(lhs, a.T @ y.to(torch.float64))
+    last = model.net[-1][0]
+    with torch.no_grad():
+        last.weight.copy_(sol[:-1].to(torch.float32)[None, :])
+        last.bias.copy_(sol[-1:].to(torch.float32))
+    return model
+
+
+def uneven_workload(n_models=24, seed=MASTER_SEED + 500, device="cpu"):
+    """Batched uneven workload: per model a width in {64, 128, 256, 512}, 1-3
+    levels, its own bumpy-sphere target (sub-seed), a grid resolution in
+    {128, 192, 256, 384, 512} and one orbit camera.  The coarse level (width x 3,
+    omega 30) gets its output layer fitted to the target; residual levels are
+    seeded (omega 60, 90; last layer x 0.1) with bands (0.05, 0.02)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(n_models):
+        width = int(rng.choice([64, 128, 256, 512]))
+        n_levels = int(rng.integers(1, 4))
+        N = int(rng.choice([128, 192, 256, 384, 512]))
+        sub = seed + 1000 * (k + 1)
+        shape = BumpySphere(seed=sub)
+        coarse = fit_output_layer(seeded_siren([width] * 3, 30.0, sub, device=device), shape, seed=sub + 1)
+        levels = [coarse]
+        for i in range(1, n_levels):
+            levels.append(seeded_siren([width] * 3, 30.0 * (i + 1), sub + 10 * i, last_scale=0.1, device=device))
+        out.append({"levels": levels, "width": width, "N": N, "deltas": list(DELTAS[:n_levels - 1]),
+                    "camera": orbit_cameras(1, seed=sub + 2)[0], "shape": shape})
+    return out
