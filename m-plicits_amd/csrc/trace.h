@@ -13,6 +13,7 @@ struct TraceArgs {
   float hit_eps, bound_radius, band_scale;
   int pixel_center, flip_y;
   int n_levels;
+  int cols;                      // column blocks per wave tile (batched tracer: per job)
   LevelDev levels[MP_MAX_LEVELS];
   float deltas[MP_MAX_LEVELS];
   unsigned long long* next_ray;  // global ray queue head

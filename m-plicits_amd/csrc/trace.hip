@@ -26,7 +26,7 @@
 #include <vector>
 
 struct mp_trace_batch {
-  int shape = 0, n_jobs = 0, nc = 1, blocks = 0;
+  int shape = 0, n_jobs = 0, blocks = 0;
   void* mem = nullptr;             // tables, counters, per-job scratch (one allocation)
   void* jobs = nullptr;            // TraceArgs [n_jobs] (device)
   void* outs = nullptr;            // TraceOut [n_jobs] (device)
@@ -259,12 +259,16 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
 // Every job in turn: a wave moves to the next job once the current one has no
 // rays left for it, so tiles never mix networks and idle waves help with the
 // jobs still running.
-template <class S, int NC>
+template <class S>
 __global__ __launch_bounds__(256) void k_trace_jobs(const TraceArgs* __restrict__ jobs, int n_jobs) {
   extern __shared__ uint4 lds_raw[];
   typename S::Frag* hand = reinterpret_cast<typename S::Frag*>(reinterpret_cast<char*>(lds_raw) +
                                                                (threadIdx.x >> 6) * TRACE_HAND_BYTES);
-  for (int j = 0; j < n_jobs; ++j) trace_rays<S, NC>(jobs[j], hand);
+  for (int j = 0; j < n_jobs; ++j) {
+    // two-block tiles unless a level of the job is 512 wide (FP32: always one)
+    if (S::ROWS == 32 && jobs[j].cols == 2) trace_rays<S, 2>(jobs[j], hand);
+    else trace_rays<S, 1>(jobs[j], hand);
+  }
 }
 
 // Compact the hit rays: ids and positions.
@@ -381,15 +385,15 @@ __global__ void k_shade_jobs(const TraceArgs* __restrict__ jobs, const TraceOut*
   }
 }
 
-template <class S, int NC>
+template <class S>
 static int launch_trace_jobs(int blocks, size_t lds, hipStream_t s, const TraceArgs* jobs, int n_jobs) {
   static bool attr_set = false;
   if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trace_jobs<S, NC>),
+    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trace_jobs<S>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_trace_jobs<S, NC>), dim3(blocks), dim3(256), lds, s, jobs, n_jobs);
+  hipLaunchKernelGGL((k_trace_jobs<S>), dim3(blocks), dim3(256), lds, s, jobs, n_jobs);
   MP_LAUNCHED("k_trace_jobs");
   return MP_OK;
 }
@@ -528,7 +532,6 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
   MP_REQUIRE(jobs != nullptr && n_jobs >= 1, MP_EINVAL, "mp_trace_batch_plan: no jobs");
   hipStream_t s = as_stream(stream);
   const int shape = jobs[0].stack ? jobs[0].stack->shape : -1;
-  int maxw = 0;
   for (int j = 0; j < n_jobs; ++j) {
     const mp_trace_job& J = jobs[j];
     const mp_trace_params* prm = &J.params;
@@ -541,12 +544,10 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
     MP_REQUIRE(prm->n_deltas == J.stack->n_levels - 1, MP_EINVAL, "mp_trace_batch_plan: job %d: need %d deltas, got %d",
                j, J.stack->n_levels - 1, prm->n_deltas);
     MP_REQUIRE((int64_t)prm->width * prm->height < (1ll << 32), MP_EUNSUPPORTED, "mp_trace_batch_plan: job %d: too many rays", j);
-    for (int l = 0; l < J.stack->n_levels; ++l) maxw = J.stack->lv[l].width > maxw ? J.stack->lv[l].width : maxw;
   }
   mp_trace_batch* p = new mp_trace_batch();
   p->shape = shape;
   p->n_jobs = n_jobs;
-  p->nc = (jobs[0].stack->nc == 2 && maxw <= 256) ? 2 : 1;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off = 0;
   const size_t off_jobs = off; off += al(sizeof(TraceArgs) * n_jobs);
@@ -587,7 +588,12 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
     a.pixel_center = J.params.pixel_center;
     a.flip_y = J.params.flip_y;
     a.n_levels = J.stack->n_levels;
-    for (int l = 0; l < a.n_levels; ++l) a.levels[l] = J.stack->lv[l].dev;
+    int jw = 0;
+    for (int l = 0; l < a.n_levels; ++l) {
+      a.levels[l] = J.stack->lv[l].dev;
+      jw = J.stack->lv[l].width > jw ? J.stack->lv[l].width : jw;
+    }
+    a.cols = (J.stack->nc == 2 && jw <= 256) ? 2 : 1;
     for (int i = 0; i < J.params.n_deltas; ++i) a.deltas[i] = J.params.deltas[i];
     unsigned long long* c = p->ctr + (size_t)j * TB_CTR;
     a.next_ray = &c[0];
@@ -633,17 +639,9 @@ int mp_trace_batch_run(mp_trace_batch* p, void* stream) {
   const size_t lds = 4 * TRACE_HAND_BYTES;
   int rc = MP_OK;
   switch (p->shape) {
-    case SH_32x32x16_BF16:
-      rc = p->nc == 2 ? launch_trace_jobs<S32x32x16_bf16, 2>(p->blocks, lds, s, jobs, p->n_jobs)
-                      : launch_trace_jobs<S32x32x16_bf16, 1>(p->blocks, lds, s, jobs, p->n_jobs);
-      break;
-    case SH_32x32x16_F16:
-      rc = p->nc == 2 ? launch_trace_jobs<S32x32x16_f16, 2>(p->blocks, lds, s, jobs, p->n_jobs)
-                      : launch_trace_jobs<S32x32x16_f16, 1>(p->blocks, lds, s, jobs, p->n_jobs);
-      break;
-    default:
-      rc = launch_trace_jobs<S16x16x4_f32, 1>(p->blocks, lds, s, jobs, p->n_jobs);
-      break;
+    case SH_32x32x16_BF16: rc = launch_trace_jobs<S32x32x16_bf16>(p->blocks, lds, s, jobs, p->n_jobs); break;
+    case SH_32x32x16_F16: rc = launch_trace_jobs<S32x32x16_f16>(p->blocks, lds, s, jobs, p->n_jobs); break;
+    default: rc = launch_trace_jobs<S16x16x4_f32>(p->blocks, lds, s, jobs, p->n_jobs); break;
   }
   if (rc != MP_OK) return rc;
   hipLaunchKernelGGL(k_collect_jobs, dim3(p->blocks * 4), dim3(256), 0, s, jobs, outs, p->n_jobs);
