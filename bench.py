@@ -44,6 +44,10 @@ COARSE = osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")
 # algorithmic work per sample of one level: 2 FLOP per multiply-add of
 # 3 -> W -> ... -> W -> 1 (first layer, hidden GEMMs, output layer)
 PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}
+# HBM bytes of one launch of the dominant kernel (coarse level over the
+# grid), from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated:
+# (2,774 + 524,288) KiB
+TRAFFIC_BYTES = (2774 + 524288) * 1024
 
 
 def level_flops(hidden):
@@ -308,9 +312,11 @@ def main():
     coarse_flop = level_flops(levels[0].hidden_widths()) * N ** 3
     k_ms = dominant_kernel_ms(levels[0], "bf16")
     achieved = coarse_flop / (k_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_level16<S32x32x16_bf16,256,2> (coarse 256x3 over 512^3)",
+    roofline = {"bound": "mfma", "kernel": "k_level<S32x32x16_bf16,256,2> (coarse 256x3 over 512^3)",
                 "achieved": round(achieved, 1), "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_TFLOPS["bf16"], 4), "traffic": None,
+                "frac": round(achieved / PEAK_TFLOPS["bf16"], 4), "traffic": TRAFFIC_BYTES,
+                "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 of this launch "
+                                  "(profiles/extract512_hbm_counters.md)",
                 "kernel_ms": round(k_ms, 3), "flop_per_launch": coarse_flop}
 
     cpu = None
