@@ -159,6 +159,16 @@ __device__ inline float sin_reduced(float a) {
 
 __device__ inline float cos_rev(float r) { return __builtin_amdgcn_cosf(r); }
 
+// sin and cos of a radian argument with one shared reduction.
+__device__ inline void sincos_reduced(float a, float& s, float& c) {
+  const float k = rintf(a * 0.159154943091895336f);
+  float r = fmaf(k, -6.28318548202514648f, a);
+  r = fmaf(k, 1.74845553146951557e-07f, r);
+  const float rev = r * 0.159154943091895336f;
+  s = __builtin_amdgcn_sinf(rev);
+  c = __builtin_amdgcn_cosf(rev);
+}
+
 __device__ inline float cos_reduced(float a) {
   const float k = rintf(a * 0.159154943091895336f);
   float r = fmaf(k, -6.28318548202514648f, a);

@@ -17,6 +17,7 @@
 // then dW_j = [abar; adbar]^T [p_{j-1}; pd_{j-1}], db_j = sum abar_j and
 // [pbar; pdbar]_{j-1} = [abar; adbar] W_j (GEMMs, issued by the host).
 #include "common.h"
+#include "mlp_core.h"
 
 namespace mp {
 
@@ -36,7 +37,7 @@ __global__ void k_train_first(const float* __restrict__ x, const float* __restri
   const float w0 = W0[3 * u], w1 = W0[3 * u + 1], w2 = W0[3 * u + 2];
   const float a = fmaf(w2, x[3 * p + 2], fmaf(w1, x[3 * p + 1], fmaf(w0, x[3 * p], b0[u])));
   float s, c;
-  sincosf(a, &s, &c);
+  sincos_reduced(a, s, c);
   const int64_t base = p * (1 + nt) * W + u;
   Z[base] = a;
   st(X, base, s);
@@ -61,7 +62,7 @@ __global__ void k_train_act(float* __restrict__ Z, const float* __restrict__ b, 
   const float a = Z[base] + b[u];
   Z[base] = a;
   float s, c;
-  sincosf(a, &s, &c);
+  sincos_reduced(a, s, c);
   st(X, base, s);
   for (int t = 0; t < nt; ++t) {
     const int64_t k = base + (int64_t)(1 + t) * W;
@@ -79,7 +80,7 @@ __global__ void k_train_act_bwd(const float* __restrict__ Z, const float* __rest
   const int u = (int)(i % W);
   const int64_t base = p * (1 + nt) * W + u;
   float s, c;
-  sincosf(Z[base], &s, &c);
+  sincos_reduced(Z[base], s, c);
   float acc = c * Pbar[base];
   for (int t = 0; t < nt; ++t) {
     const int64_t k = base + (int64_t)(1 + t) * W;
@@ -104,7 +105,126 @@ __global__ void k_train_seed(const float* __restrict__ s, const float* __restric
   for (int t = 0; t < nt; ++t) Pbar[base + (int64_t)(1 + t) * W] = g ? g[p * nt + t] * w : 0.f;
 }
 
+// Four consecutive units per thread (W % 4 == 0): 16-byte loads of the FP32
+// blocks, 8- or 16-byte stores of the GEMM operands.  NT = tangent rows.
+template <class T> struct Vec4;
+template <> struct Vec4<float> { typedef float4 type; };
+template <> struct Vec4<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(4))); };
+
+template <class T>
+__device__ inline void st4(T* p, int64_t i, float4 v) {
+  typename Vec4<T>::type o;
+  o[0] = (T)v.x; o[1] = (T)v.y; o[2] = (T)v.z; o[3] = (T)v.w;
+  *reinterpret_cast<typename Vec4<T>::type*>(p + i) = o;
+}
+template <>
+__device__ inline void st4<float>(float* p, int64_t i, float4 v) { *reinterpret_cast<float4*>(p + i) = v; }
+__device__ inline float4 ld4(const float* p, int64_t i) { return *reinterpret_cast<const float4*>(p + i); }
+
+template <class T, int NT>
+__global__ __launch_bounds__(256) void k_train_first4(const float* __restrict__ x, const float* __restrict__ v,
+                                                      const float* __restrict__ W0, const float* __restrict__ b0,
+                                                      int64_t n, int W, float* __restrict__ Z, T* __restrict__ X) {
+  const int W4 = W / 4;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * W4) return;
+  const int64_t p = q / W4;
+  const int u = (int)(q - p * W4) * 4;
+  const int64_t base = p * (1 + NT) * W + u;
+  const float x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
+  float a[4], s[4], c[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float* w = W0 + 3 * (u + e);
+    a[e] = fmaf(w[2], x2, fmaf(w[1], x1, fmaf(w[0], x0, b0[u + e])));
+    sincos_reduced(a[e], s[e], c[e]);
+  }
+  *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
+  st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const float* vt = v + (p * NT + t) * 3;
+    float ad[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float* w = W0 + 3 * (u + e);
+      ad[e] = fmaf(w[2], vt[2], fmaf(w[1], vt[1], w[0] * vt[0]));
+    }
+    const int64_t k = base + (int64_t)(1 + t) * W;
+    *reinterpret_cast<float4*>(Z + k) = make_float4(ad[0], ad[1], ad[2], ad[3]);
+    st4(X, k, make_float4(c[0] * ad[0], c[1] * ad[1], c[2] * ad[2], c[3] * ad[3]));
+  }
+}
+
+template <class T, int NT>
+__global__ __launch_bounds__(256) void k_train_act4(float* __restrict__ Z, const float* __restrict__ b, int64_t n,
+                                                    int W, T* __restrict__ X) {
+  const int W4 = W / 4;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * W4) return;
+  const int64_t p = q / W4;
+  const int u = (int)(q - p * W4) * 4;
+  const int64_t base = p * (1 + NT) * W + u;
+  const float4 z = ld4(Z, base);
+  const float4 bb = *reinterpret_cast<const float4*>(b + u);
+  const float a[4] = {z.x + bb.x, z.y + bb.y, z.z + bb.z, z.w + bb.w};
+  float s[4], c[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) sincos_reduced(a[e], s[e], c[e]);
+  *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
+  st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int64_t k = base + (int64_t)(1 + t) * W;
+    const float4 ad = ld4(Z, k);
+    st4(X, k, make_float4(c[0] * ad.x, c[1] * ad.y, c[2] * ad.z, c[3] * ad.w));
+  }
+}
+
+template <class T, int NT>
+__global__ __launch_bounds__(256) void k_train_act_bwd4(const float* __restrict__ Z, const float* __restrict__ Pbar,
+                                                        int64_t n, int W, T* __restrict__ Abar,
+                                                        float* __restrict__ abar_val) {
+  const int W4 = W / 4;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * W4) return;
+  const int64_t p = q / W4;
+  const int u = (int)(q - p * W4) * 4;
+  const int64_t base = p * (1 + NT) * W + u;
+  const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
+  const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
+  float s[4], c[4], acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    sincos_reduced(za[e], s[e], c[e]);
+    acc[e] = c[e] * pa[e];
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int64_t k = base + (int64_t)(1 + t) * W;
+    const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
+    const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = fmaf(-s[e] * zv[e], pv[e], acc[e]);
+    st4(Abar, k, make_float4(c[0] * pv[0], c[1] * pv[1], c[2] * pv[2], c[3] * pv[3]));
+  }
+  st4(Abar, base, make_float4(acc[0], acc[1], acc[2], acc[3]));
+  if (abar_val) *reinterpret_cast<float4*>(abar_val + p * W + u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+}
+
 static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// Launch K<T, NT> for the run-time tangent count.
+#define MP_TRAIN_NT_DISPATCH(KERNEL, T, nt, grid, s, ...)                                     \
+  do {                                                                                        \
+    switch (nt) {                                                                             \
+      case 0: hipLaunchKernelGGL((KERNEL<T, 0>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
+      case 1: hipLaunchKernelGGL((KERNEL<T, 1>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<T, 2>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<T, 3>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
+    }                                                                                         \
+  } while (0)
 
 }  // namespace mp
 
@@ -119,6 +239,18 @@ int mp_train_first(const float* x, const float* v, const float* W0, const float*
   MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first: tangent directions are NULL");
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
+  if (W % 4 == 0) {
+    const unsigned g = grid_for(n * (W / 4));
+    if (x_dtype == 1)
+      MP_TRAIN_NT_DISPATCH(k_train_first4, __bf16, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<__bf16*>(X));
+    else if (x_dtype == 2)
+      MP_TRAIN_NT_DISPATCH(k_train_first4, _Float16, n_tangents, g, s, x, v, W0, b0, n, W, Z,
+                           reinterpret_cast<_Float16*>(X));
+    else
+      MP_TRAIN_NT_DISPATCH(k_train_first4, float, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<float*>(X));
+    MP_LAUNCHED("k_train_first4");
+    return MP_OK;
+  }
   if (x_dtype == 1)
     hipLaunchKernelGGL(k_train_first<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
                        n_tangents, Z, reinterpret_cast<__bf16*>(X));
@@ -137,6 +269,17 @@ int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, voi
              "mp_train_act: bad arguments");
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
+  if (W % 4 == 0) {
+    const unsigned g = grid_for(n * (W / 4));
+    if (x_dtype == 1)
+      MP_TRAIN_NT_DISPATCH(k_train_act4, __bf16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X));
+    else if (x_dtype == 2)
+      MP_TRAIN_NT_DISPATCH(k_train_act4, _Float16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<_Float16*>(X));
+    else
+      MP_TRAIN_NT_DISPATCH(k_train_act4, float, n_tangents, g, s, Z, b, n, W, reinterpret_cast<float*>(X));
+    MP_LAUNCHED("k_train_act4");
+    return MP_OK;
+  }
   if (x_dtype == 1)
     hipLaunchKernelGGL(k_train_act<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
                        reinterpret_cast<__bf16*>(X));
@@ -156,6 +299,20 @@ int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_
              "mp_train_act_bwd: bad arguments");
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
+  if (W % 4 == 0) {
+    const unsigned g = grid_for(n * (W / 4));
+    if (a_dtype == 1)
+      MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, __bf16, n_tangents, g, s, Z, Pbar, n, W, reinterpret_cast<__bf16*>(Abar),
+                           abar_val);
+    else if (a_dtype == 2)
+      MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, _Float16, n_tangents, g, s, Z, Pbar, n, W,
+                           reinterpret_cast<_Float16*>(Abar), abar_val);
+    else
+      MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, float, n_tangents, g, s, Z, Pbar, n, W, reinterpret_cast<float*>(Abar),
+                           abar_val);
+    MP_LAUNCHED("k_train_act_bwd4");
+    return MP_OK;
+  }
   if (a_dtype == 1)
     hipLaunchKernelGGL(k_train_act_bwd<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W, n_tangents,
                        reinterpret_cast<__bf16*>(Abar), abar_val);
@@ -202,7 +359,7 @@ __global__ __launch_bounds__(256) void k_train_out_fwd(const float* __restrict__
   const float* z = Z + p * (1 + nt) * W;
   for (int u = lane; u < W; u += 64) {
     float s, c;
-    sincosf(z[u], &s, &c);
+    sincos_reduced(z[u], s, c);
     const float w = wl[u];
     acc[0] = fmaf(w, s, acc[0]);
     for (int t = 0; t < nt; ++t) acc[1 + t] = fmaf(w * c, z[(1 + t) * W + u], acc[1 + t]);
@@ -231,7 +388,7 @@ __global__ __launch_bounds__(256) void k_train_out_bwd(const float* __restrict__
       const float* z = Z + p * (1 + nt) * W;
       float* pb = Pbar + p * (1 + nt) * W;
       float sn, cs;
-      sincosf(z[u], &sn, &cs);
+      sincos_reduced(z[u], sn, cs);
       const float sv = s ? s[p] : 0.f;
       pb[u] = sv * w;
       acc = fmaf(sv, sn, acc);

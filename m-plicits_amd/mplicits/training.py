@@ -136,9 +136,10 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
     aval0 = torch.empty(n, w0, device=x.device, dtype=torch.float32)
     _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[0]), _lib.ptr(Pbar), n, w0, nt, _lib.ptr(A0), 0, _lib.ptr(aval0),
                                     stream))
-    dW0 = A0[:, 0].t() @ x
-    if nt:
-        dW0 = dW0 + torch.einsum("ntw,ntc->wc", A0[:, 1:], V)
+    # dW0 = sum over value and tangent rows of A0 (x) [x | v_t]: one GEMM over
+    # the contiguous [n (1 + T), w0] block
+    X0 = x[:, None, :] if not nt else torch.cat([x[:, None, :], V], 1)
+    dW0 = torch.mm(A0.view(-1, w0).t(), X0.reshape(-1, 3))
     grads[0] = (dW0, aval0.sum(0))
     return grads
 
