@@ -76,11 +76,21 @@ int mp_stack_update(mp_stack* stack, void* stream);
 void mp_stack_destroy(mp_stack* stack);
 int mp_stack_levels(const mp_stack* stack);
 
-/* Points per wavefront tile of the level kernels (a tile shape picked by the
- * autotuner, mplicits/autotune.py): 1 or 2 column blocks of the MFMA shape.
+/* Points per wavefront tile of the level kernels (a tile shape picked by
+ * mp_autotune): 1 or 2 column blocks of the MFMA shape (2 = the default
+ * block count of each width).
  * No counterpart in the reference (its CUTLASS tile shapes are compile-time,
  * renderer/cuda/layer_gemms.cu:1612-1856). */
 int mp_stack_set_tiles(mp_stack* stack, int column_blocks);
+
+/* Autotuner of that knob: measures 1 and 2 column blocks for a seeded level
+ * 3 -> width x n_hidden -> 1 in `precision` on a 160^3 grid (HIP events, after a
+ * warm-up) and writes the faster one to *column_blocks.  cache_path (may be
+ * NULL) is a text file of "width n_hidden precision column_blocks" lines: a
+ * cached entry is returned without measuring, a new one is appended.
+ * Allocates device memory; host-sync point; serialised by a process-wide
+ * mutex.  No counterpart in the reference. */
+int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, int* column_blocks);
 
 /* Evaluate sum_{l=lvl0}^{lvl1-1} f_l(x) at n points xyz [n][3] into sdf [n].
  * Replaces SIREN.forward (i3d/model.py:107-129) summed over levels as

@@ -93,6 +93,8 @@ SIGNATURES = {
     "mp_stack_destroy": (None, [ctypes.c_void_p]),
     "mp_stack_levels": (ctypes.c_int, [ctypes.c_void_p]),
     "mp_stack_set_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mp_autotune": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int)]),
     "mp_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mp_extract_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,

@@ -92,3 +92,18 @@ def test_trace_batch_matches_single_model_trace():
             assert torch.equal(b.normals[j], ref["normals"]), f"run {rep}, job {j}"
             assert st[j]["hits"] == ref["hits"] and st[j]["evals"] == ref["evals"]
     b.close()
+
+
+def test_autotune_caches_and_applies(tmp_path):
+    from mplicits import autotune
+    cache = str(tmp_path / "tiles.txt")
+    cols = autotune.best_tiles(256, 3, "bf16", cache=cache)
+    assert cols in (1, 2)
+    assert open(cache).read().split() == ["256", "3", "1", str(cols)]
+    assert autotune.best_tiles(256, 3, "bf16", cache=cache) == cols  # cached: no second line
+    assert len(open(cache).read().splitlines()) == 1
+    st = LevelStack(_model(256, 2, 9), "bf16")
+    assert autotune.tune_stack(st, cache=cache) == cols
+    x = torch.rand(4096, 3, device="cuda") * 2 - 1
+    ref = LevelStack(_model(256, 2, 9), "bf16").eval(x)
+    assert torch.equal(st.eval(x), ref)  # the tile shape changes the schedule, not the values
