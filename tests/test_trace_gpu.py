@@ -80,3 +80,22 @@ def test_stack_matches_oracle_step_rule():
     assert both.sum() > 50
     assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
     assert out["evals"][0] > 0 and out["evals"][1] > 0
+
+
+def test_render_cli_benchmark_line(tmp_path, capsys):
+    import importlib.util
+    import os.path as osp
+    from mplicits import renderer_io
+    from mplicits.synthetic import seeded_siren
+    levels = [seeded_siren([64] * 3, 30.0, 1, last_scale=20.0), seeded_siren([64] * 2, 60.0, 2, last_scale=0.1),
+              seeded_siren([64] * 2, 90.0, 3, last_scale=0.1)]
+    desc = renderer_io.export_experiment(levels, str(tmp_path / "data" / "m"), "m", data_root="data")
+    spec = importlib.util.spec_from_file_location(
+        "render_cli", osp.join(osp.dirname(__file__), "..", "m-plicits_amd", "render.py"))
+    render = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(render)
+    render.main([f"-experiment_file={desc}", f"-data_root={tmp_path}", "-resolution=64x48", "-benchmark=2",
+                 f"-out={tmp_path / 'img.ppm'}"])
+    out = capsys.readouterr().out
+    assert "BENCHMARK resolution=64x48" in out and "frames=2 avg_fps=" in out
+    assert open(tmp_path / "img.ppm", "rb").read(11) == b"P6 64 48 25"
