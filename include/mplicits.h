@@ -224,6 +224,13 @@ int mp_train_seed(const float* s, const float* g, const float* w_last, int64_t n
 int mp_train_out_fwd(const float* Z, const float* w_last, const float* b_last, int64_t n, int W,
                      int n_tangents, float* f, float* fd, void* stream);
 int64_t mp_train_out_bwd_blocks(int64_t n);
+/* Layer-0 backward fused with its parameter gradients (W a multiple of 4,
+ * at most 512): part [mp_train_first_bwd_blocks(n)][W][4] receives per-block
+ * partial sums of (dW_0[u][0..2], db_0[u]) = sum over points of
+ * abar x + sum_t adbar_t v_t and abar, with abar / adbar as in mp_train_act_bwd. */
+int64_t mp_train_first_bwd_blocks(int64_t n);
+int mp_train_first_bwd(const float* Z, const float* Pbar, const float* x, const float* v, int64_t n, int W,
+                       int n_tangents, float* part, void* stream);
 int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
                      int n_tangents, float* Pbar, float* dwl_part, void* stream);
 

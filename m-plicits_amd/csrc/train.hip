@@ -402,6 +402,107 @@ __global__ __launch_bounds__(256) void k_train_out_bwd(const float* __restrict__
   }
 }
 
+// Rows (points) per block of the fused reductions below.
+constexpr int RED_ROWS = 64;
+
+// Output layer backward, four units per thread (W % 4 == 0, blockDim = W / 4).
+template <int NT>
+__global__ __launch_bounds__(128) void k_train_out_bwd4(const float* __restrict__ Z, const float* __restrict__ s,
+                                                        const float* __restrict__ g, const float* __restrict__ wl,
+                                                        int64_t n, int W, float* __restrict__ Pbar,
+                                                        float* __restrict__ dwl_part) {
+  const int u = threadIdx.x * 4;
+  if (u >= W) return;
+  const float4 w4 = ld4(wl, u);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
+  for (int r = 0; r < RED_ROWS; ++r) {
+    const int64_t p = p0 + r;
+    if (p >= n) break;
+    const int64_t base = p * (1 + NT) * W + u;
+    const float4 z = ld4(Z, base);
+    const float za[4] = {z.x, z.y, z.z, z.w};
+    float sn[4], cs[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) sincos_reduced(za[e], sn[e], cs[e]);
+    const float sv = s ? s[p] : 0.f;
+    *reinterpret_cast<float4*>(Pbar + base) = make_float4(sv * w4.x, sv * w4.y, sv * w4.z, sv * w4.w);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = fmaf(sv, sn[e], acc[e]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int64_t k = base + (int64_t)(1 + t) * W;
+      const float gv = g ? g[p * NT + t] : 0.f;
+      *reinterpret_cast<float4*>(Pbar + k) = make_float4(gv * w4.x, gv * w4.y, gv * w4.z, gv * w4.w);
+      const float4 zt = ld4(Z, k);
+      const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = fmaf(gv * cs[e], zv[e], acc[e]);
+    }
+  }
+  *reinterpret_cast<float4*>(dwl_part + (int64_t)blockIdx.x * W + u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+}
+
+// Layer-0 backward fused with its parameter gradients (W % 4 == 0,
+// blockDim = W / 4): abar = cos a pbar - sin a sum_t ad_t pdbar_t and
+// adbar_t = cos a pdbar_t are reduced in place against the layer input
+// [x; v_t], giving per-block partial sums part [blocks][W][4] of
+// (dW0[u][0..2], db0[u]).
+template <int NT>
+__global__ __launch_bounds__(128) void k_train_first_bwd4(const float* __restrict__ Z, const float* __restrict__ Pbar,
+                                                          const float* __restrict__ x, const float* __restrict__ v,
+                                                          int64_t n, int W, float* __restrict__ part) {
+  const int u = threadIdx.x * 4;
+  if (u >= W) return;
+  float acc[4][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[e][k] = 0.f;
+  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
+  for (int r = 0; r < RED_ROWS; ++r) {
+    const int64_t p = p0 + r;
+    if (p >= n) break;
+    const int64_t base = p * (1 + NT) * W + u;
+    const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
+    const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
+    float sn[4], cs[4], abar[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sincos_reduced(za[e], sn[e], cs[e]);
+      abar[e] = cs[e] * pa[e];
+    }
+    const float x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int64_t k = base + (int64_t)(1 + t) * W;
+      const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
+      const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
+      const float* vt = v + (p * NT + t) * 3;
+      const float v0 = vt[0], v1 = vt[1], v2 = vt[2];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        abar[e] = fmaf(-sn[e] * zv[e], pv[e], abar[e]);
+        const float ad = cs[e] * pv[e];
+        acc[e][0] = fmaf(ad, v0, acc[e][0]);
+        acc[e][1] = fmaf(ad, v1, acc[e][1]);
+        acc[e][2] = fmaf(ad, v2, acc[e][2]);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      acc[e][0] = fmaf(abar[e], x0, acc[e][0]);
+      acc[e][1] = fmaf(abar[e], x1, acc[e][1]);
+      acc[e][2] = fmaf(abar[e], x2, acc[e][2]);
+      acc[e][3] += abar[e];
+    }
+  }
+  float* o = part + ((int64_t)blockIdx.x * W + u) * 4;
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    *reinterpret_cast<float4*>(o + 4 * e) = make_float4(acc[e][0], acc[e][1], acc[e][2], acc[e][3]);
+}
+
 }  // namespace mp
 
 extern "C" {
@@ -418,7 +519,7 @@ int mp_train_out_fwd(const float* Z, const float* w_last, const float* b_last, i
   return MP_OK;
 }
 
-int64_t mp_train_out_bwd_blocks(int64_t n) { return (n + 255) / 256; }
+int64_t mp_train_out_bwd_blocks(int64_t n) { return (n + RED_ROWS - 1) / RED_ROWS; }
 
 int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
                      int n_tangents, float* Pbar, float* dwl_part, void* stream) {
@@ -426,9 +527,41 @@ int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float
              "mp_train_out_bwd: bad arguments");
   if (n == 0) return MP_OK;
   const int64_t blocks = mp_train_out_bwd_blocks(n);
-  hipLaunchKernelGGL(k_train_out_bwd, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), Z, s, g, w_last, n,
-                     W, n_tangents, 256, Pbar, dwl_part);
+  hipStream_t st = as_stream(stream);
+  if (W % 4 == 0 && W <= 512) {
+    switch (n_tangents) {
+      case 0: hipLaunchKernelGGL(k_train_out_bwd4<0>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
+      case 1: hipLaunchKernelGGL(k_train_out_bwd4<1>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
+      case 2: hipLaunchKernelGGL(k_train_out_bwd4<2>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
+      default: hipLaunchKernelGGL(k_train_out_bwd4<3>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
+    }
+    MP_LAUNCHED("k_train_out_bwd4");
+    return MP_OK;
+  }
+  hipLaunchKernelGGL(k_train_out_bwd, dim3((unsigned)blocks), dim3(256), 0, st, Z, s, g, w_last, n,
+                     W, n_tangents, RED_ROWS, Pbar, dwl_part);
   MP_LAUNCHED("k_train_out_bwd");
+  return MP_OK;
+}
+
+int64_t mp_train_first_bwd_blocks(int64_t n) { return (n + RED_ROWS - 1) / RED_ROWS; }
+
+int mp_train_first_bwd(const float* Z, const float* Pbar, const float* x, const float* v, int64_t n, int W,
+                       int n_tangents, float* part, void* stream) {
+  MP_REQUIRE(Z && Pbar && x && part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_first_bwd: bad arguments");
+  MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first_bwd: tangent directions are NULL");
+  MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED, "mp_train_first_bwd: width %d (needs a multiple of 4, <= 512)", W);
+  if (n == 0) return MP_OK;
+  const int64_t blocks = mp_train_first_bwd_blocks(n);
+  hipStream_t st = as_stream(stream);
+  switch (n_tangents) {
+    case 0: hipLaunchKernelGGL(k_train_first_bwd4<0>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
+    case 1: hipLaunchKernelGGL(k_train_first_bwd4<1>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
+    case 2: hipLaunchKernelGGL(k_train_first_bwd4<2>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
+    default: hipLaunchKernelGGL(k_train_first_bwd4<3>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
+  }
+  MP_LAUNCHED("k_train_first_bwd4");
   return MP_OK;
 }
 

@@ -132,6 +132,15 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
         grads[j] = (dW, aval.sum(0))
         Pbar = _mm(A2, _gemm_weight(Wj, precision), precision).view(n, 1 + nt, win)
     w0 = Zs[0].shape[-1]
+    if w0 % 4 == 0 and w0 <= 512:
+        # layer-0 adjoints reduced against [x; v_t] in the same pass
+        part = torch.empty(int(lib.mp_train_first_bwd_blocks(n)), w0, 4, device=x.device, dtype=torch.float32)
+        _lib.check(lib.mp_train_first_bwd(_lib.ptr(Zs[0]), _lib.ptr(Pbar), _lib.ptr(x),
+                                          _lib.ptr(V.contiguous() if nt else None), n, w0, nt, _lib.ptr(part),
+                                          stream))
+        tot = part.sum(0)
+        grads[0] = (tot[:, :3].contiguous(), tot[:, 3].contiguous())
+        return grads
     A0 = torch.empty(n, 1 + nt, w0, device=x.device, dtype=torch.float32)
     aval0 = torch.empty(n, w0, device=x.device, dtype=torch.float32)
     _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[0]), _lib.ptr(Pbar), n, w0, nt, _lib.ptr(A0), 0, _lib.ptr(aval0),
