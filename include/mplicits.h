@@ -210,11 +210,21 @@ void mp_trace_batch_destroy(mp_trace_batch* plan);
  *                     abar_val [n][W] (float32, may be NULL) = value row
  *   mp_train_seed:    Pbar = [s w_L; g_t w_L] (s or g may be NULL = 0) */
 int mp_train_first(const float* x, const float* v, const float* W0, const float* b0, int64_t n, int W,
-                   int n_tangents, float* Z, void* X, int x_dtype, void* stream);
-int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype,
+                   int n_tangents, float* Z, void* X, int x_dtype, void* Xb, void* stream);
+int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype, void* Xb,
                  void* stream);
 int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
-                     int a_dtype, float* abar_val, void* stream);
+                     int a_dtype, float* abar_val, void* Ab, void* stream);
+/* Xb / Ab (may be NULL): a second, K-blocked copy of X / Abar for
+ * mp_train_dw (16-bit, three tangents, even n, W % 4 == 0): element
+ * (row r = 4 p + t, unit u) at ((r / 8) * W + u) * 8 + r % 8.
+ * mp_train_dw: split-K weight gradient dW [Wout][Win] = sum_rows A[r][i] X[r][j]
+ * from the blocked copies (rows = 4 n, a multiple of 64; widths multiples of
+ * 256): part [mp_train_dw_chunks(rows, Wout, Win)][Wout][Win] FP32 partial
+ * sums, to be added by the caller. */
+int mp_train_dw_chunks(int64_t rows, int Wout, int Win);
+int mp_train_dw(const void* Ab, const void* Xb, int64_t rows, int Wout, int Win, int dtype, float* part,
+                void* stream);
 int mp_train_seed(const float* s, const float* g, const float* w_last, int64_t n, int W, int n_tangents,
                   float* Pbar, void* stream);
 /* Output layer (W -> 1) from the last sine layer's float32 pre-activations Z:

@@ -122,10 +122,26 @@ template <>
 __device__ inline void st4<float>(float* p, int64_t i, float4 v) { *reinterpret_cast<float4*>(p + i) = v; }
 __device__ inline float4 ld4(const float* p, int64_t i) { return *reinterpret_cast<const float4*>(p + i); }
 
+// K-blocked copy of a [n][4][W] operand block (three tangents), the layout of
+// the weight-gradient GEMM: element (row r = 4 p + t, unit u) at
+// ((r / 8) * W + u) * 8 + r % 8, so eight consecutive rows of one unit are 16
+// contiguous bytes.  vals[t][e]: rows t = 0..3 of point p, units u..u+3.
+template <class T>
+__device__ inline void st_blocked(T* Xb, int64_t p, int W, int u, const float (&vals)[4][4]) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v4 o;
+    o[0] = (T)vals[0][e]; o[1] = (T)vals[1][e]; o[2] = (T)vals[2][e]; o[3] = (T)vals[3][e];
+    *reinterpret_cast<v4*>(Xb + ((p >> 1) * W + u + e) * 8 + 4 * (p & 1)) = o;
+  }
+}
+
 template <class T, int NT>
 __global__ __launch_bounds__(256) void k_train_first4(const float* __restrict__ x, const float* __restrict__ v,
                                                       const float* __restrict__ W0, const float* __restrict__ b0,
-                                                      int64_t n, int W, float* __restrict__ Z, T* __restrict__ X) {
+                                                      int64_t n, int W, float* __restrict__ Z, T* __restrict__ X,
+                                                      T* __restrict__ Xb) {
   const int W4 = W / 4;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * W4) return;
@@ -142,6 +158,9 @@ __global__ __launch_bounds__(256) void k_train_first4(const float* __restrict__ 
   }
   *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
   st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
+  float rows[4][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) rows[0][e] = s[e];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const float* vt = v + (p * NT + t) * 3;
@@ -154,12 +173,19 @@ __global__ __launch_bounds__(256) void k_train_first4(const float* __restrict__ 
     const int64_t k = base + (int64_t)(1 + t) * W;
     *reinterpret_cast<float4*>(Z + k) = make_float4(ad[0], ad[1], ad[2], ad[3]);
     st4(X, k, make_float4(c[0] * ad[0], c[1] * ad[1], c[2] * ad[2], c[3] * ad[3]));
+    if constexpr (NT == 3) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rows[1 + t][e] = c[e] * ad[e];
+    }
+  }
+  if constexpr (NT == 3) {
+    if (Xb) st_blocked(Xb, p, W, u, rows);
   }
 }
 
 template <class T, int NT>
 __global__ __launch_bounds__(256) void k_train_act4(float* __restrict__ Z, const float* __restrict__ b, int64_t n,
-                                                    int W, T* __restrict__ X) {
+                                                    int W, T* __restrict__ X, T* __restrict__ Xb) {
   const int W4 = W / 4;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * W4) return;
@@ -174,18 +200,29 @@ __global__ __launch_bounds__(256) void k_train_act4(float* __restrict__ Z, const
   for (int e = 0; e < 4; ++e) sincos_reduced(a[e], s[e], c[e]);
   *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
   st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
+  float rows[4][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) rows[0][e] = s[e];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int64_t k = base + (int64_t)(1 + t) * W;
     const float4 ad = ld4(Z, k);
-    st4(X, k, make_float4(c[0] * ad.x, c[1] * ad.y, c[2] * ad.z, c[3] * ad.w));
+    const float o[4] = {c[0] * ad.x, c[1] * ad.y, c[2] * ad.z, c[3] * ad.w};
+    st4(X, k, make_float4(o[0], o[1], o[2], o[3]));
+    if constexpr (NT == 3) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+    }
+  }
+  if constexpr (NT == 3) {
+    if (Xb) st_blocked(Xb, p, W, u, rows);
   }
 }
 
 template <class T, int NT>
 __global__ __launch_bounds__(256) void k_train_act_bwd4(const float* __restrict__ Z, const float* __restrict__ Pbar,
                                                         int64_t n, int W, T* __restrict__ Abar,
-                                                        float* __restrict__ abar_val) {
+                                                        float* __restrict__ abar_val, T* __restrict__ Ab) {
   const int W4 = W / 4;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * W4) return;
@@ -200,6 +237,7 @@ __global__ __launch_bounds__(256) void k_train_act_bwd4(const float* __restrict_
     sincos_reduced(za[e], s[e], c[e]);
     acc[e] = c[e] * pa[e];
   }
+  float rows[4][4];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int64_t k = base + (int64_t)(1 + t) * W;
@@ -207,10 +245,22 @@ __global__ __launch_bounds__(256) void k_train_act_bwd4(const float* __restrict_
     const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
 #pragma unroll
     for (int e = 0; e < 4; ++e) acc[e] = fmaf(-s[e] * zv[e], pv[e], acc[e]);
-    st4(Abar, k, make_float4(c[0] * pv[0], c[1] * pv[1], c[2] * pv[2], c[3] * pv[3]));
+    const float o[4] = {c[0] * pv[0], c[1] * pv[1], c[2] * pv[2], c[3] * pv[3]};
+    st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
+    if constexpr (NT == 3) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+    }
   }
   st4(Abar, base, make_float4(acc[0], acc[1], acc[2], acc[3]));
   if (abar_val) *reinterpret_cast<float4*>(abar_val + p * W + u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  if constexpr (NT == 3) {
+    if (Ab) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rows[0][e] = acc[e];
+      st_blocked(Ab, p, W, u, rows);
+    }
+  }
 }
 
 static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -233,21 +283,25 @@ using namespace mp;
 extern "C" {
 
 int mp_train_first(const float* x, const float* v, const float* W0, const float* b0, int64_t n, int W,
-                   int n_tangents, float* Z, void* X, int x_dtype, void* stream) {
+                   int n_tangents, float* Z, void* X, int x_dtype, void* Xb, void* stream) {
   MP_REQUIRE(x && W0 && b0 && Z && X && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
              "mp_train_first: bad arguments");
   MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first: tangent directions are NULL");
+  MP_REQUIRE(!Xb || (W % 4 == 0 && n_tangents == 3 && n % 2 == 0 && x_dtype != 0), MP_EINVAL,
+             "mp_train_first: the blocked copy needs 16-bit X, 3 tangents, even n and W % 4 == 0");
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
   if (W % 4 == 0) {
     const unsigned g = grid_for(n * (W / 4));
     if (x_dtype == 1)
-      MP_TRAIN_NT_DISPATCH(k_train_first4, __bf16, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<__bf16*>(X));
+      MP_TRAIN_NT_DISPATCH(k_train_first4, __bf16, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<__bf16*>(X),
+                           reinterpret_cast<__bf16*>(Xb));
     else if (x_dtype == 2)
       MP_TRAIN_NT_DISPATCH(k_train_first4, _Float16, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                           reinterpret_cast<_Float16*>(X));
+                           reinterpret_cast<_Float16*>(X), reinterpret_cast<_Float16*>(Xb));
     else
-      MP_TRAIN_NT_DISPATCH(k_train_first4, float, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<float*>(X));
+      MP_TRAIN_NT_DISPATCH(k_train_first4, float, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<float*>(X),
+                           static_cast<float*>(nullptr));
     MP_LAUNCHED("k_train_first4");
     return MP_OK;
   }
@@ -264,19 +318,25 @@ int mp_train_first(const float* x, const float* v, const float* W0, const float*
   return MP_OK;
 }
 
-int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype, void* stream) {
+int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype, void* Xb,
+                 void* stream) {
   MP_REQUIRE(Z && b && X && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
              "mp_train_act: bad arguments");
+  MP_REQUIRE(!Xb || (W % 4 == 0 && n_tangents == 3 && n % 2 == 0 && x_dtype != 0), MP_EINVAL,
+             "mp_train_act: the blocked copy needs 16-bit X, 3 tangents, even n and W % 4 == 0");
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
   if (W % 4 == 0) {
     const unsigned g = grid_for(n * (W / 4));
     if (x_dtype == 1)
-      MP_TRAIN_NT_DISPATCH(k_train_act4, __bf16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X));
+      MP_TRAIN_NT_DISPATCH(k_train_act4, __bf16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X),
+                           reinterpret_cast<__bf16*>(Xb));
     else if (x_dtype == 2)
-      MP_TRAIN_NT_DISPATCH(k_train_act4, _Float16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<_Float16*>(X));
+      MP_TRAIN_NT_DISPATCH(k_train_act4, _Float16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<_Float16*>(X),
+                           reinterpret_cast<_Float16*>(Xb));
     else
-      MP_TRAIN_NT_DISPATCH(k_train_act4, float, n_tangents, g, s, Z, b, n, W, reinterpret_cast<float*>(X));
+      MP_TRAIN_NT_DISPATCH(k_train_act4, float, n_tangents, g, s, Z, b, n, W, reinterpret_cast<float*>(X),
+                           static_cast<float*>(nullptr));
     MP_LAUNCHED("k_train_act4");
     return MP_OK;
   }
@@ -294,22 +354,24 @@ int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, voi
 }
 
 int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
-                     int a_dtype, float* abar_val, void* stream) {
+                     int a_dtype, float* abar_val, void* Ab, void* stream) {
   MP_REQUIRE(Z && Pbar && Abar && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
              "mp_train_act_bwd: bad arguments");
+  MP_REQUIRE(!Ab || (W % 4 == 0 && n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
+             "mp_train_act_bwd: the blocked copy needs 16-bit A, 3 tangents, even n and W % 4 == 0");
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
   if (W % 4 == 0) {
     const unsigned g = grid_for(n * (W / 4));
     if (a_dtype == 1)
       MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, __bf16, n_tangents, g, s, Z, Pbar, n, W, reinterpret_cast<__bf16*>(Abar),
-                           abar_val);
+                           abar_val, reinterpret_cast<__bf16*>(Ab));
     else if (a_dtype == 2)
       MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, _Float16, n_tangents, g, s, Z, Pbar, n, W,
-                           reinterpret_cast<_Float16*>(Abar), abar_val);
+                           reinterpret_cast<_Float16*>(Abar), abar_val, reinterpret_cast<_Float16*>(Ab));
     else
       MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, float, n_tangents, g, s, Z, Pbar, n, W, reinterpret_cast<float*>(Abar),
-                           abar_val);
+                           abar_val, static_cast<float*>(nullptr));
     MP_LAUNCHED("k_train_act_bwd4");
     return MP_OK;
   }

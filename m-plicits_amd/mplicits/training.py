@@ -60,11 +60,18 @@ def unit_tangents(n, device):
 
 
 @torch.no_grad()
+def _blocked_dw(precision, n, nt, wout, win):
+    """Whether the weight gradient of a wout x win hidden layer runs as the
+    split-K kernel on K-blocked operand copies (16-bit, 3 tangents)."""
+    return (precision != "fp32" and nt == 3 and n % 16 == 0 and wout % 256 == 0 and win % 256 == 0)
+
+
 def forward_tangents(layers, x, V, precision="fp32"):
     """f [n] and its directional derivatives fd [n, T] along V [n, T, 3].
 
     Returns (f, fd, cache); the cache holds the pre-activations Z_j (FP32,
-    [n, 1+T, W]) and the GEMM inputs X_j of every layer for the backward."""
+    [n, 1+T, W]) and the GEMM inputs X_j of every layer for the backward (and,
+    where the blocked weight-gradient kernel applies, their K-blocked copies)."""
     lib = _lib.load()
     n = x.shape[0]
     nt = 0 if V is None else V.shape[1]
@@ -72,21 +79,32 @@ def forward_tangents(layers, x, V, precision="fp32"):
     stream = _lib.stream_handle()
     W0, b0, _ = layers[0]
     w = W0.shape[0]
+    hidden = [lay[0].shape for lay in layers[1:-1]]  # (wout, win) of the hidden GEMMs
+
+    def blocked(j, width):
+        # X_j feeds hidden GEMM j + 1
+        if j < len(hidden) and _blocked_dw(precision, n, nt, hidden[j][0], width):
+            return torch.empty(n * (1 + nt) * width, device=x.device, dtype=dt)
+        return None
+
     Z = torch.empty(n, 1 + nt, w, device=x.device, dtype=torch.float32)
     X = torch.empty(n, 1 + nt, w, device=x.device, dtype=dt)
+    Xb = blocked(0, w)
     _lib.check(lib.mp_train_first(_lib.ptr(x), _lib.ptr(V), _lib.ptr(W0.detach().contiguous()),
                                   _lib.ptr(b0.detach().contiguous()), n, w, nt, _lib.ptr(Z), _lib.ptr(X),
-                                  _CODE[precision], stream))
-    Zs, Xs = [Z], [X]
-    for Wj, bj, _ in layers[1:-1]:
+                                  _CODE[precision], _lib.ptr(Xb), stream))
+    Zs, Xs, Xbs = [Z], [X], [Xb]
+    for j, (Wj, bj, _) in enumerate(layers[1:-1], start=1):
         wout = Wj.shape[0]
         Zj = _mm(Xs[-1].view(-1, Xs[-1].shape[-1]), _gemm_weight(Wj.detach(), precision).t(), precision)
         Zj = Zj.view(n, 1 + nt, wout)
         Xj = torch.empty(n, 1 + nt, wout, device=x.device, dtype=dt)
+        Xbj = blocked(j, wout)
         _lib.check(lib.mp_train_act(_lib.ptr(Zj), _lib.ptr(bj.detach().contiguous()), n, wout, nt, _lib.ptr(Xj),
-                                    _CODE[precision], stream))
+                                    _CODE[precision], _lib.ptr(Xbj), stream))
         Zs.append(Zj)
         Xs.append(Xj)
+        Xbs.append(Xbj)
     WL, bL, _ = layers[-1]
     # output layer in FP32 from the FP32 pre-activations of the last sine layer
     f = torch.empty(n, device=x.device, dtype=torch.float32)
@@ -94,7 +112,7 @@ def forward_tangents(layers, x, V, precision="fp32"):
     _lib.check(lib.mp_train_out_fwd(_lib.ptr(Zs[-1]), _lib.ptr(WL.detach().contiguous()),
                                     _lib.ptr(bL.detach().contiguous()), n, Zs[-1].shape[-1], nt, _lib.ptr(f),
                                     _lib.ptr(fd if nt else None), stream))
-    return f, fd, (Zs, Xs)
+    return f, fd, (Zs, Xs, Xbs)
 
 
 @torch.no_grad()
@@ -102,7 +120,7 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
     """Parameter gradients of sum_i [s_i f_i + sum_t g_it fd_it] (effective,
     frequency-baked parameters).  Returns [(dW_k, db_k)] per layer."""
     lib = _lib.load()
-    Zs, Xs = cache
+    Zs, Xs, Xbs = cache
     n = x.shape[0]
     nt = Zs[0].shape[1] - 1
     dt = _DT[precision]
@@ -125,10 +143,19 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
         wout, win = Wj.shape
         A = torch.empty(n, 1 + nt, wout, device=x.device, dtype=dt)
         aval = torch.empty(n, wout, device=x.device, dtype=torch.float32)
+        Ab = torch.empty_like(A) if Xbs[j - 1] is not None else None
         _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[j]), _lib.ptr(Pbar), n, wout, nt, _lib.ptr(A),
-                                        _CODE[precision], _lib.ptr(aval), stream))
+                                        _CODE[precision], _lib.ptr(aval), _lib.ptr(Ab), stream))
         A2 = A.view(-1, wout)
-        dW = _mm(A2.t(), Xs[j - 1].view(-1, win), precision)
+        if Ab is not None:
+            rows = n * (1 + nt)
+            part = torch.empty(int(lib.mp_train_dw_chunks(rows, wout, win)), wout, win, device=x.device,
+                               dtype=torch.float32)
+            _lib.check(lib.mp_train_dw(_lib.ptr(Ab), _lib.ptr(Xbs[j - 1]), rows, wout, win, _CODE[precision],
+                                       _lib.ptr(part), stream))
+            dW = part.sum(0)
+        else:
+            dW = _mm(A2.t(), Xs[j - 1].view(-1, win), precision)
         grads[j] = (dW, aval.sum(0))
         Pbar = _mm(A2, _gemm_weight(Wj, precision), precision).view(n, 1 + nt, win)
     w0 = Zs[0].shape[-1]
@@ -144,7 +171,7 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
     A0 = torch.empty(n, 1 + nt, w0, device=x.device, dtype=torch.float32)
     aval0 = torch.empty(n, w0, device=x.device, dtype=torch.float32)
     _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[0]), _lib.ptr(Pbar), n, w0, nt, _lib.ptr(A0), 0, _lib.ptr(aval0),
-                                    stream))
+                                    None, stream))
     # dW0 = sum over value and tangent rows of A0 (x) [x | v_t]: one GEMM over
     # the contiguous [n (1 + T), w0] block
     X0 = x[:, None, :] if not nt else torch.cat([x[:, None, :], V], 1)

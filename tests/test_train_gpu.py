@@ -102,3 +102,58 @@ def test_forward_tangents_are_the_input_gradient():
     fr, gr = osiren.forward_grad(osiren.layers_from_model(fine), x.double().cpu().numpy())
     assert np.abs(f.double().cpu().numpy() - fr).max() < 1e-5 * max(1, np.abs(fr).max())
     assert np.abs(fd.double().cpu().numpy() - gr).max() < 1e-4 * np.abs(gr).max()
+
+
+def _blocked(t):
+    """[rows, W] -> K-blocked copy: element (r, u) at ((r / 8) * W + u) * 8 + r % 8."""
+    rows, w = t.shape
+    return t.view(rows // 8, 8, w).transpose(1, 2).contiguous().view(-1)
+
+
+@pytest.mark.parametrize("dtype,code", [(torch.bfloat16, 1), (torch.float16, 2)])
+@pytest.mark.parametrize("rows,wout,win", [(64, 256, 256), (12224, 512, 256),
+                                           (1 << 16, 256, 512)])
+def test_split_k_weight_gradient_matches_fp32_gemm(dtype, code, rows, wout, win):
+    from mplicits import _lib
+    lib = _lib.load()
+    g = torch.Generator(device="cuda").manual_seed(rows + wout)
+    A = torch.randn(rows, wout, device="cuda", generator=g).to(dtype)
+    X = torch.randn(rows, win, device="cuda", generator=g).to(dtype)
+    chunks = int(lib.mp_train_dw_chunks(rows, wout, win))
+    part = torch.full((chunks, wout, win), float("nan"), device="cuda")
+    Ab, Xb = _blocked(A), _blocked(X)  # held until the kernel has run
+    _lib.check(lib.mp_train_dw(_lib.ptr(Ab), _lib.ptr(Xb), rows, wout, win, code,
+                               _lib.ptr(part), _lib.stream_handle()))
+    torch.cuda.synchronize()
+    got = part.sum(0)
+    ref = A.float().t() @ X.float()
+    # same products, FP32 accumulation in a different order
+    bound = 1e-4 * (A.float().abs().t() @ X.float().abs()) + 1e-6
+    assert torch.all((got - ref).abs() <= bound), float(((got - ref).abs() / bound).max())
+
+
+def test_split_k_weight_gradient_rejects_ragged_rows():
+    from mplicits import _lib
+    lib = _lib.load()
+    A = torch.zeros(96, 256, device="cuda", dtype=torch.bfloat16)
+    part = torch.empty(int(lib.mp_train_dw_chunks(96, 256, 256)), 256, 256, device="cuda")
+    with pytest.raises(ValueError, match="multiple of 64"):
+        _lib.check(lib.mp_train_dw(_lib.ptr(A), _lib.ptr(A), 96, 256, 256, 1, _lib.ptr(part), _lib.stream_handle()))
+
+
+def test_wide_level_gradients_take_the_blocked_path():
+    coarse, med, fine = _models((256, 256, 256))
+    x, gt, nrm = _batch(2048, 3)
+    assert training._blocked_dw("bf16", 2048, 3, 256, 256)
+    ref_loss, ref = _oracle_grads(coarse, med, fine, x, gt, nrm)
+    fine.precision = "bf16"
+    for m in (coarse, med):
+        m.precision = "bf16"
+    loss_fn = loss_functions.loss_add_detail_fine(coarse, med)
+    losses = loss_fn(fine(x.cuda()), {"sdf": gt.cuda(), "normals": nrm.cuda()})
+    loss = sum(losses.values())
+    fine.zero_grad()
+    loss.backward()
+    for p, r in zip(fine.parameters(), ref):
+        err = (p.grad.double().cpu() - r).norm() / r.norm()
+        assert err < 6e-2, (tuple(p.shape), float(err))
