@@ -228,12 +228,27 @@ int mp_train_dw(const void* Ab, const void* Xb, int64_t rows, int Wout, int Win,
 int mp_train_seed(const float* s, const float* g, const float* w_last, int64_t n, int W, int n_tangents,
                   float* Pbar, void* stream);
 /* Output layer (W -> 1) from the last sine layer's float32 pre-activations Z:
- * forward f [n] = w_L . sin a + b_L, fd [n][T] = w_L . (cos a ad_t);
- * backward Pbar = [s w_L; g_t w_L] and per-block partial sums dwl_part
- * [mp_train_out_bwd_blocks(n)][W] of dw_L = sum s sin a + sum_t g_t cos a ad_t. */
-int mp_train_out_fwd(const float* Z, const float* w_last, const float* b_last, int64_t n, int W,
+ * forward f [n] = w_L . sin a + b_L, fd [n][T] = w_L . (cos a ad_t), with
+ * b_sine (may be NULL) = the last sine layer's bias, added to Z's value row in
+ * place first (so that layer needs no mp_train_act: its X is never a GEMM
+ * operand); backward Pbar = [s w_L; g_t w_L] and per-block partial sums
+ * dwl_part [mp_train_out_bwd_blocks(n)][W] of
+ * dw_L = sum s sin a + sum_t g_t cos a ad_t. */
+int mp_train_out_fwd(float* Z, const float* b_sine, const float* w_last, const float* b_last, int64_t n, int W,
                      int n_tangents, float* f, float* fd, void* stream);
 int64_t mp_train_out_bwd_blocks(int64_t n);
+/* Output layer and last sine layer backward in one pass (W a multiple of 4,
+ * at most 512; Pbar is never stored): Abar (and its blocked copy Ab, may be
+ * NULL) as mp_train_act_bwd computes it from Pbar = [s w_L; g_t w_L], and
+ * part [mp_train_out_bwd_blocks(n)][2][W] per-block partial sums of dw_L
+ * (row 0) and the sine layer's db = sum abar (row 1). */
+int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
+                         int n_tangents, void* Abar, int a_dtype, void* Ab, float* part, void* stream);
+/* mp_train_act_bwd with the bias gradient as per-block partial sums
+ * db_part [mp_train_out_bwd_blocks(n)][W] instead of the [n][W] value rows
+ * (W a multiple of 4, at most 512). */
+int mp_train_act_bwd_part(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
+                          int a_dtype, void* Ab, float* db_part, void* stream);
 /* Layer-0 backward fused with its parameter gradients (W a multiple of 4,
  * at most 512): part [mp_train_first_bwd_blocks(n)][W][4] receives per-block
  * partial sums of (dW_0[u][0..2], db_0[u]) = sum over points of

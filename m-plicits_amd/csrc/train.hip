@@ -411,17 +411,25 @@ int mp_train_seed(const float* s_val, const float* g_tan, const float* wl, int64
 // ---------------------------------------------------------------------------
 namespace mp {
 
-__global__ __launch_bounds__(256) void k_train_out_fwd(const float* __restrict__ Z, const float* __restrict__ wl,
-                                                       const float* __restrict__ bl, int64_t n, int W, int nt,
-                                                       float* __restrict__ f, float* __restrict__ fd) {
+// b_sine (may be NULL): bias of the last sine layer, added to the value row
+// of Z in place first (Z then holds that layer's full pre-activations).
+__global__ __launch_bounds__(256) void k_train_out_fwd(float* __restrict__ Z, const float* __restrict__ b_sine,
+                                                       const float* __restrict__ wl, const float* __restrict__ bl,
+                                                       int64_t n, int W, int nt, float* __restrict__ f,
+                                                       float* __restrict__ fd) {
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (p >= n) return;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const float* z = Z + p * (1 + nt) * W;
+  float* z = Z + p * (1 + nt) * W;
   for (int u = lane; u < W; u += 64) {
+    float a = z[u];
+    if (b_sine) {
+      a += b_sine[u];
+      z[u] = a;
+    }
     float s, c;
-    sincos_reduced(z[u], s, c);
+    sincos_reduced(a, s, c);
     const float w = wl[u];
     acc[0] = fmaf(w, s, acc[0]);
     for (int t = 0; t < nt; ++t) acc[1 + t] = fmaf(w * c, z[(1 + t) * W + u], acc[1 + t]);
@@ -505,6 +513,131 @@ __global__ __launch_bounds__(128) void k_train_out_bwd4(const float* __restrict_
   *reinterpret_cast<float4*>(dwl_part + (int64_t)blockIdx.x * W + u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
 }
 
+// Output layer and last sine layer backward in one pass (W % 4 == 0,
+// blockDim = W / 4).  The seeds of the last sine layer are rank one,
+// pbar = s w_L and pdbar_t = g_t w_L, so they are formed in registers:
+//   abar = cos a pbar - sin a sum_t ad_t pdbar_t,  adbar_t = cos a pdbar_t
+// are written as the GEMM operand (and its K-blocked copy), and
+// part [blocks][2][W] receives per-block partial sums of
+// dw_L = sum s sin a + sum_t g_t cos a ad_t (row 0) and db = sum abar (row 1).
+template <class T, int NT>
+__global__ __launch_bounds__(128) void k_train_out_act_bwd4(const float* __restrict__ Z, const float* __restrict__ s,
+                                                            const float* __restrict__ g, const float* __restrict__ wl,
+                                                            int64_t n, int W, T* __restrict__ Abar, T* __restrict__ Ab,
+                                                            float* __restrict__ part) {
+  const int u = threadIdx.x * 4;
+  if (u >= W) return;
+  const float4 w4 = ld4(wl, u);
+  const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+  float dwl[4] = {0.f, 0.f, 0.f, 0.f}, db[4] = {0.f, 0.f, 0.f, 0.f};
+  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
+  for (int r = 0; r < RED_ROWS; ++r) {
+    const int64_t p = p0 + r;
+    if (p >= n) break;
+    const int64_t base = p * (1 + NT) * W + u;
+    const float4 z = ld4(Z, base);
+    const float za[4] = {z.x, z.y, z.z, z.w};
+    float sn[4], cs[4], abar[4];
+    const float sv = s ? s[p] : 0.f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sincos_reduced(za[e], sn[e], cs[e]);
+      abar[e] = cs[e] * (sv * wv[e]);
+      dwl[e] = fmaf(sv, sn[e], dwl[e]);
+    }
+    float rows[4][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int64_t k = base + (int64_t)(1 + t) * W;
+      const float gv = g ? g[p * NT + t] : 0.f;
+      const float4 zt = ld4(Z, k);
+      const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
+      float o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float pt = gv * wv[e];
+        abar[e] = fmaf(-sn[e] * zv[e], pt, abar[e]);
+        o[e] = cs[e] * pt;
+        dwl[e] = fmaf(gv * cs[e], zv[e], dwl[e]);
+      }
+      st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
+      if constexpr (NT == 3) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+      }
+    }
+    st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) db[e] += abar[e];
+    if constexpr (NT == 3) {
+      if (Ab) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rows[0][e] = abar[e];
+        st_blocked(Ab, p, W, u, rows);
+      }
+    }
+  }
+  float* o = part + (int64_t)blockIdx.x * 2 * W + u;
+  *reinterpret_cast<float4*>(o) = make_float4(dwl[0], dwl[1], dwl[2], dwl[3]);
+  *reinterpret_cast<float4*>(o + W) = make_float4(db[0], db[1], db[2], db[3]);
+}
+
+// Hidden-layer backward through sin / the tangent products (as
+// k_train_act_bwd4) with the bias gradient reduced per block: blockDim =
+// W / 4, db_part [blocks][W].
+template <class T, int NT>
+__global__ __launch_bounds__(128) void k_train_act_bwd_part4(const float* __restrict__ Z,
+                                                             const float* __restrict__ Pbar, int64_t n, int W,
+                                                             T* __restrict__ Abar, T* __restrict__ Ab,
+                                                             float* __restrict__ db_part) {
+  const int u = threadIdx.x * 4;
+  if (u >= W) return;
+  float db[4] = {0.f, 0.f, 0.f, 0.f};
+  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
+  for (int r = 0; r < RED_ROWS; ++r) {
+    const int64_t p = p0 + r;
+    if (p >= n) break;
+    const int64_t base = p * (1 + NT) * W + u;
+    const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
+    const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
+    float sn[4], cs[4], abar[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sincos_reduced(za[e], sn[e], cs[e]);
+      abar[e] = cs[e] * pa[e];
+    }
+    float rows[4][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int64_t k = base + (int64_t)(1 + t) * W;
+      const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
+      const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
+      float o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        abar[e] = fmaf(-sn[e] * zv[e], pv[e], abar[e]);
+        o[e] = cs[e] * pv[e];
+      }
+      st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
+      if constexpr (NT == 3) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+      }
+    }
+    st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) db[e] += abar[e];
+    if constexpr (NT == 3) {
+      if (Ab) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rows[0][e] = abar[e];
+        st_blocked(Ab, p, W, u, rows);
+      }
+    }
+  }
+  *reinterpret_cast<float4*>(db_part + (int64_t)blockIdx.x * W + u) = make_float4(db[0], db[1], db[2], db[3]);
+}
+
 // Layer-0 backward fused with its parameter gradients (W % 4 == 0,
 // blockDim = W / 4): abar = cos a pbar - sin a sum_t ad_t pdbar_t and
 // adbar_t = cos a pdbar_t are reduced in place against the layer input
@@ -569,14 +702,14 @@ __global__ __launch_bounds__(128) void k_train_first_bwd4(const float* __restric
 
 extern "C" {
 
-int mp_train_out_fwd(const float* Z, const float* w_last, const float* b_last, int64_t n, int W, int n_tangents,
-                     float* f, float* fd, void* stream) {
+int mp_train_out_fwd(float* Z, const float* b_sine, const float* w_last, const float* b_last, int64_t n, int W,
+                     int n_tangents, float* f, float* fd, void* stream) {
   MP_REQUIRE(Z && w_last && b_last && f && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
              "mp_train_out_fwd: bad arguments");
   MP_REQUIRE(n_tangents == 0 || fd, MP_EINVAL, "mp_train_out_fwd: fd is NULL");
   if (n == 0) return MP_OK;
-  hipLaunchKernelGGL(k_train_out_fwd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, as_stream(stream), Z, w_last,
-                     b_last, n, W, n_tangents, f, fd);
+  hipLaunchKernelGGL(k_train_out_fwd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, as_stream(stream), Z, b_sine,
+                     w_last, b_last, n, W, n_tangents, f, fd);
   MP_LAUNCHED("k_train_out_fwd");
   return MP_OK;
 }
@@ -603,6 +736,67 @@ int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float
   hipLaunchKernelGGL(k_train_out_bwd, dim3((unsigned)blocks), dim3(256), 0, st, Z, s, g, w_last, n,
                      W, n_tangents, RED_ROWS, Pbar, dwl_part);
   MP_LAUNCHED("k_train_out_bwd");
+  return MP_OK;
+}
+
+// Launch K<T, NT> over `blocks` blocks of W / 4 threads.
+#define MP_TRAIN_RED_DISPATCH(KERNEL, T, nt, blocks, W, s, ...)                                              \
+  do {                                                                                                      \
+    switch (nt) {                                                                                           \
+      case 0: hipLaunchKernelGGL((KERNEL<T, 0>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+      case 1: hipLaunchKernelGGL((KERNEL<T, 1>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<T, 2>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<T, 3>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+    }                                                                                                       \
+  } while (0)
+
+int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
+                         int n_tangents, void* Abar, int a_dtype, void* Ab, float* part, void* stream) {
+  MP_REQUIRE(Z && w_last && Abar && part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_out_act_bwd: bad arguments");
+  MP_REQUIRE(a_dtype >= 0 && a_dtype <= 2, MP_EINVAL, "mp_train_out_act_bwd: a_dtype %d", a_dtype);
+  MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED, "mp_train_out_act_bwd: width %d (needs a multiple of 4, <= 512)",
+             W);
+  MP_REQUIRE(!Ab || (n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
+             "mp_train_out_act_bwd: the blocked copy needs 16-bit A, 3 tangents and even n");
+  if (n == 0) return MP_OK;
+  const int64_t blocks = mp_train_out_bwd_blocks(n);
+  hipStream_t st = as_stream(stream);
+  if (a_dtype == 1)
+    MP_TRAIN_RED_DISPATCH(k_train_out_act_bwd4, __bf16, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                          reinterpret_cast<__bf16*>(Abar), reinterpret_cast<__bf16*>(Ab), part);
+  else if (a_dtype == 2)
+    MP_TRAIN_RED_DISPATCH(k_train_out_act_bwd4, _Float16, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                          reinterpret_cast<_Float16*>(Abar), reinterpret_cast<_Float16*>(Ab), part);
+  else
+    MP_TRAIN_RED_DISPATCH(k_train_out_act_bwd4, float, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                          reinterpret_cast<float*>(Abar), static_cast<float*>(nullptr), part);
+  MP_LAUNCHED("k_train_out_act_bwd4");
+  return MP_OK;
+}
+
+int mp_train_act_bwd_part(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
+                          int a_dtype, void* Ab, float* db_part, void* stream) {
+  MP_REQUIRE(Z && Pbar && Abar && db_part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
+             "mp_train_act_bwd_part: bad arguments");
+  MP_REQUIRE(a_dtype >= 0 && a_dtype <= 2, MP_EINVAL, "mp_train_act_bwd_part: a_dtype %d", a_dtype);
+  MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED,
+             "mp_train_act_bwd_part: width %d (needs a multiple of 4, <= 512)", W);
+  MP_REQUIRE(!Ab || (n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
+             "mp_train_act_bwd_part: the blocked copy needs 16-bit A, 3 tangents and even n");
+  if (n == 0) return MP_OK;
+  const int64_t blocks = mp_train_out_bwd_blocks(n);
+  hipStream_t st = as_stream(stream);
+  if (a_dtype == 1)
+    MP_TRAIN_RED_DISPATCH(k_train_act_bwd_part4, __bf16, n_tangents, blocks, W, st, Z, Pbar, n, W,
+                          reinterpret_cast<__bf16*>(Abar), reinterpret_cast<__bf16*>(Ab), db_part);
+  else if (a_dtype == 2)
+    MP_TRAIN_RED_DISPATCH(k_train_act_bwd_part4, _Float16, n_tangents, blocks, W, st, Z, Pbar, n, W,
+                          reinterpret_cast<_Float16*>(Abar), reinterpret_cast<_Float16*>(Ab), db_part);
+  else
+    MP_TRAIN_RED_DISPATCH(k_train_act_bwd_part4, float, n_tangents, blocks, W, st, Z, Pbar, n, W,
+                          reinterpret_cast<float*>(Abar), static_cast<float*>(nullptr), db_part);
+  MP_LAUNCHED("k_train_act_bwd_part4");
   return MP_OK;
 }
 

@@ -94,10 +94,19 @@ def forward_tangents(layers, x, V, precision="fp32"):
                                   _lib.ptr(b0.detach().contiguous()), n, w, nt, _lib.ptr(Z), _lib.ptr(X),
                                   _CODE[precision], _lib.ptr(Xb), stream))
     Zs, Xs, Xbs = [Z], [X], [Xb]
+    b_sine = None
     for j, (Wj, bj, _) in enumerate(layers[1:-1], start=1):
         wout = Wj.shape[0]
         Zj = _mm(Xs[-1].view(-1, Xs[-1].shape[-1]), _gemm_weight(Wj.detach(), precision).t(), precision)
         Zj = Zj.view(n, 1 + nt, wout)
+        if j == len(layers) - 2:
+            # the last sine layer's X is no GEMM operand: the output layer
+            # reads Z and adds this layer's bias to it in place
+            b_sine = bj.detach().contiguous()
+            Zs.append(Zj)
+            Xs.append(None)
+            Xbs.append(None)
+            break
         Xj = torch.empty(n, 1 + nt, wout, device=x.device, dtype=dt)
         Xbj = blocked(j, wout)
         _lib.check(lib.mp_train_act(_lib.ptr(Zj), _lib.ptr(bj.detach().contiguous()), n, wout, nt, _lib.ptr(Xj),
@@ -109,7 +118,7 @@ def forward_tangents(layers, x, V, precision="fp32"):
     # output layer in FP32 from the FP32 pre-activations of the last sine layer
     f = torch.empty(n, device=x.device, dtype=torch.float32)
     fd = torch.empty(n, nt, device=x.device, dtype=torch.float32)
-    _lib.check(lib.mp_train_out_fwd(_lib.ptr(Zs[-1]), _lib.ptr(WL.detach().contiguous()),
+    _lib.check(lib.mp_train_out_fwd(_lib.ptr(Zs[-1]), _lib.ptr(b_sine), _lib.ptr(WL.detach().contiguous()),
                                     _lib.ptr(bL.detach().contiguous()), n, Zs[-1].shape[-1], nt, _lib.ptr(f),
                                     _lib.ptr(fd if nt else None), stream))
     return f, fd, (Zs, Xs, Xbs)
@@ -129,23 +138,21 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
     wl = WL.detach()[0].contiguous()
     zl = Zs[-1]
     wlast = zl.shape[-1]
-    # output layer: seeds of the last sine layer and dw_L in one pass
-    Pbar = torch.empty(n, 1 + nt, wlast, device=x.device, dtype=torch.float32)
-    part = torch.empty(int(lib.mp_train_out_bwd_blocks(n)), wlast, device=x.device, dtype=torch.float32)
-    _lib.check(lib.mp_train_out_bwd(_lib.ptr(zl), _lib.ptr(s), _lib.ptr(g.contiguous() if g is not None else None),
-                                    _lib.ptr(wl), n, wlast, nt, _lib.ptr(Pbar), _lib.ptr(part), stream))
-    dwl = part.sum(0)
+    L = len(layers)
+    nblk = int(lib.mp_train_out_bwd_blocks(n))
+    gc = g.contiguous() if g is not None else None
     dbl = s.sum().reshape(1) if s is not None else torch.zeros(1, device=x.device)
-    grads = [None] * len(layers)
-    grads[-1] = (dwl.reshape(1, -1), dbl)
-    for j in range(len(layers) - 2, 0, -1):
+    grads = [None] * L
+
+    def adjoint(j):
+        # Abar_j and, where the split-K kernel applies, its K-blocked copy
+        A = torch.empty(n, 1 + nt, layers[j][0].shape[0], device=x.device, dtype=dt)
+        return A, (torch.empty_like(A) if Xbs[j - 1] is not None else None)
+
+    def hidden_bwd(j, A, Ab, db):
+        # dW_j from Abar_j and X_{j-1}; returns the seeds [pbar; pdbar]_{j-1}
         Wj = layers[j][0].detach()
         wout, win = Wj.shape
-        A = torch.empty(n, 1 + nt, wout, device=x.device, dtype=dt)
-        aval = torch.empty(n, wout, device=x.device, dtype=torch.float32)
-        Ab = torch.empty_like(A) if Xbs[j - 1] is not None else None
-        _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[j]), _lib.ptr(Pbar), n, wout, nt, _lib.ptr(A),
-                                        _CODE[precision], _lib.ptr(aval), _lib.ptr(Ab), stream))
         A2 = A.view(-1, wout)
         if Ab is not None:
             rows = n * (1 + nt)
@@ -156,8 +163,42 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
             dW = part.sum(0)
         else:
             dW = _mm(A2.t(), Xs[j - 1].view(-1, win), precision)
-        grads[j] = (dW, aval.sum(0))
-        Pbar = _mm(A2, _gemm_weight(Wj, precision), precision).view(n, 1 + nt, win)
+        grads[j] = (dW, db)
+        return _mm(A2, _gemm_weight(Wj, precision), precision).view(n, 1 + nt, win)
+
+    if L >= 3 and wlast % 4 == 0 and wlast <= 512:
+        # output layer and last sine layer in one pass: the rank-one seeds
+        # [s w_L; g_t w_L] are never stored
+        A, Ab = adjoint(L - 2)
+        part = torch.empty(nblk, 2, wlast, device=x.device, dtype=torch.float32)
+        _lib.check(lib.mp_train_out_act_bwd(_lib.ptr(zl), _lib.ptr(s), _lib.ptr(gc), _lib.ptr(wl), n, wlast, nt,
+                                            _lib.ptr(A), _CODE[precision], _lib.ptr(Ab), _lib.ptr(part), stream))
+        tot = part.sum(0)
+        grads[-1] = (tot[0].reshape(1, -1), dbl)
+        Pbar = hidden_bwd(L - 2, A, Ab, tot[1])
+        first = L - 3
+    else:
+        # output layer: seeds of the last sine layer and dw_L in one pass
+        Pbar = torch.empty(n, 1 + nt, wlast, device=x.device, dtype=torch.float32)
+        part = torch.empty(nblk, wlast, device=x.device, dtype=torch.float32)
+        _lib.check(lib.mp_train_out_bwd(_lib.ptr(zl), _lib.ptr(s), _lib.ptr(gc), _lib.ptr(wl), n, wlast, nt,
+                                        _lib.ptr(Pbar), _lib.ptr(part), stream))
+        grads[-1] = (part.sum(0).reshape(1, -1), dbl)
+        first = L - 2
+    for j in range(first, 0, -1):
+        wout = layers[j][0].shape[0]
+        A, Ab = adjoint(j)
+        if wout % 4 == 0 and wout <= 512:
+            dbp = torch.empty(nblk, wout, device=x.device, dtype=torch.float32)
+            _lib.check(lib.mp_train_act_bwd_part(_lib.ptr(Zs[j]), _lib.ptr(Pbar), n, wout, nt, _lib.ptr(A),
+                                                 _CODE[precision], _lib.ptr(Ab), _lib.ptr(dbp), stream))
+            db = dbp.sum(0)
+        else:
+            aval = torch.empty(n, wout, device=x.device, dtype=torch.float32)
+            _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[j]), _lib.ptr(Pbar), n, wout, nt, _lib.ptr(A),
+                                            _CODE[precision], _lib.ptr(aval), _lib.ptr(Ab), stream))
+            db = aval.sum(0)
+        Pbar = hidden_bwd(j, A, Ab, db)
     w0 = Zs[0].shape[-1]
     if w0 % 4 == 0 and w0 <= 512:
         # layer-0 adjoints reduced against [x; v_t] in the same pass

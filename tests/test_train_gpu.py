@@ -49,9 +49,11 @@ def _oracle_grads(coarse, med, fine, x, gt, nrm):
     return loss.item(), [g * oms[i // 2] for i, g in enumerate(grads)]
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 2e-3), ("bf16", 6e-2)])
-def test_dropin_residual_loss_gradients(precision, tol):
-    coarse, med, fine = _models()
+@pytest.mark.parametrize("precision,tol,fine_hidden", [("fp32", 2e-3, (64, 64, 64)), ("bf16", 6e-2, (64, 64, 64)),
+                                                       ("fp32", 2e-3, (50, 30, 50))])
+def test_dropin_residual_loss_gradients(precision, tol, fine_hidden):
+    # (50, 30, 50): widths that are no multiple of 4 take the scalar kernels
+    coarse, med, fine = _models(fine_hidden)
     for m in (coarse, med, fine):
         m.precision = precision
     x, gt, nrm = _batch(2048, 1)
