@@ -171,6 +171,45 @@ int mp_trace(mp_stack* stack, const float* cams, int n_cams, const mp_trace_para
              float* depth, float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats,
              void* stream);
 
+/* The renderer's fixed-iteration frame (MIPplicit::SdfInference,
+ * renderer/cuda/state.cu:816-1055, with InitializeRays_kernel,
+ * IterateSphereTracing_kernel, sum_residual_normals and Shade_kernel of
+ * renderer/cuda/kernels.cu:28-487; its -iters / -delta / -threshold /
+ * -normal_lod / -shading / -skip_lod0 / -no_residual options,
+ * renderer/cuda/renderer.cu:754-790).  Every pixel takes iters[0] steps on
+ * level 0 (by f0 - delta), iters[1] on levels 0 + 1 and iters[2] on levels
+ * 0 (+ 1) + 2 (by the residual sum where f0 < band_scale * delta, by f0
+ * elsewhere); pixels whose last evaluated distance is below threshold and
+ * whose position is inside [-1, 1]^3 are shaded with the normal of levels
+ * 0..normal_lod (one level for standalone models), the rest are white.
+ * Pixel (i, j) of camera k is k*W*H + j*W + i; NDC 2i/W - 1 (no pixel centre).
+ * Outputs (device): rgba [rays] RGBA8 (truncated as RgbaFloatToInt does),
+ * pos [rays][3] final positions, dist [rays] shading distances, normals
+ * [rays][3] (unit where shaded, else 0); pos, dist and normals may be NULL.
+ * stats (host, may be NULL; then a host-sync point): level evaluations
+ * performed [3], shaded pixels.  With stats NULL the call only launches
+ * kernels (device scratch grows on the first call of a size), so a frame can
+ * be captured in a hipGraph. */
+#define MP_SHADE_NORMALS 0
+#define MP_SHADE_PHONG 1
+typedef struct mp_render_params {
+  int width, height;
+  int iters[3];         /* steps of the level-0, level-1 and level-2 phases (20/0/0 default, 20/5/5 full) */
+  float delta;          /* lod_0_delta: the band of level 0 */
+  float threshold;      /* distance_threshold (0.05) */
+  int normal_lod;       /* lod_to_show: levels whose normal shades the pixel */
+  int shading;          /* MP_SHADE_PHONG or MP_SHADE_NORMALS */
+  int residual;         /* 1: finer levels are residuals (the default); 0: standalone SDFs */
+  int skip_lod0;        /* 1: no level-0 phase */
+  int flip_y;
+  int lod2_reads_lod0;  /* 1: level 2's distances read level 0's output, as state.cu:834 binds them */
+  float bound_radius;   /* 1.2 */
+  float band_scale;     /* 1.4 */
+} mp_render_params;
+
+int mp_render(mp_stack* stack, const float* cams, int n_cams, const mp_render_params* params, uint32_t* rgba,
+              float* pos, float* dist, float* normals, int64_t* stats, void* stream);
+
 /* Batched tracing of many stacks, one camera each (uneven widths, levels and
  * image sizes; every stack in the same precision): the persistent tracer walks
  * the jobs in order, a wave moving on to the next job once the current one has

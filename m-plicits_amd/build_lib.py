@@ -30,6 +30,7 @@ HIP_FLAGS = [
 FILE_FLAGS = {
     "level.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "trace.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "render.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
 }
 

@@ -19,9 +19,7 @@
 //   bounding sphere or runs out of steps.
 // Rays start where they enter the sphere of radius bound_radius
 // (InitializeRays_kernel, kernels.cu:400-487).
-#include "eval_dispatch.h"
-#include "level_core.h"
-#include "trace.h"
+#include "trace_core.h"
 
 #include <vector>
 
@@ -41,64 +39,6 @@ namespace mp {
 __global__ void k_fill_ull(unsigned long long* p, int n, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
-}
-
-// LDS handoff bytes per wave: every width the tracer runs (NC * width <= 512)
-// fits this region.
-constexpr size_t TRACE_HAND_BYTES = 32 * 1024;
-
-// f_level at the NC points of this lane; hand: the wave's handoff region.
-template <class S, int NC>
-__device__ __forceinline__ void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
-                                      typename S::Frag* hand) {
-  switch (L.width) {
-    case 64: LevelCore<S, 64, NC, false>::eval(L, x, f, nullptr, hand); break;
-    case 128: LevelCore<S, 128, NC, false>::eval(L, x, f, nullptr, hand); break;
-    case 256: LevelCore<S, 256, NC, false>::eval(L, x, f, nullptr, hand); break;
-    default:
-      if constexpr (NC == 1) LevelCore<S, 512, 1, false>::eval(L, x, f, nullptr, hand);
-      break;
-  }
-}
-
-__device__ inline void ray_init(const TraceArgs& a, int64_t rid, float o[3], float d[3], float& t0,
-                                bool& inside) {
-  const int64_t per_cam = (int64_t)a.width * a.height;
-  const int cam = (int)(rid / per_cam);
-  const int64_t pix = rid % per_cam;
-  const int i = (int)(pix % a.width), j = (int)(pix / a.width);
-  const float* iv = a.cams + cam * 32;
-  const float* ip = iv + 16;
-  const float off = a.pixel_center ? 0.5f : 0.0f;
-  const float nx = (2.0f * (i + off)) / a.width - 1.0f;
-  float ny = (2.0f * (j + off)) / a.height - 1.0f;
-  if (a.flip_y) ny = -ny;
-  const float clip[4] = {nx, ny, -1.0f, 1.0f};
-  float eye[4];
-  for (int k = 0; k < 4; ++k)
-    eye[k] = ip[0 * 4 + k] * clip[0] + ip[1 * 4 + k] * clip[1] + ip[2 * 4 + k] * clip[2] + ip[3 * 4 + k] * clip[3];
-  float world[3];
-  for (int k = 0; k < 3; ++k)
-    world[k] = iv[0 * 4 + k] * eye[0] + iv[1 * 4 + k] * eye[1] + iv[2 * 4 + k] * eye[2] + iv[12 + k];
-  o[0] = iv[12]; o[1] = iv[13]; o[2] = iv[14];
-  d[0] = world[0] - o[0]; d[1] = world[1] - o[1]; d[2] = world[2] - o[2];
-  const float inv = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  d[0] *= inv; d[1] *= inv; d[2] *= inv;
-  // advance to the bounding sphere
-  const float R = a.bound_radius;
-  const float b = o[0] * d[0] + o[1] * d[1] + o[2] * d[2];
-  const float c = o[0] * o[0] + o[1] * o[1] + o[2] * o[2] - R * R;
-  t0 = 0.f;
-  inside = true;
-  if (c > 0.f) {
-    const float disc = b * b - c;
-    if (b < 0.f && disc > 0.f) {
-      t0 = -b - sqrtf(disc);
-      o[0] += t0 * d[0]; o[1] += t0 * d[1]; o[2] += t0 * d[2];
-    } else {
-      inside = false;  // the ray never meets the bounding sphere
-    }
-  }
 }
 
 // Trace the rays of one job until its queue is empty and this wave's slots
@@ -290,11 +230,6 @@ __global__ void k_collect_hits(const float* depth, const float* hit_pos, int64_t
     pos[3 * k + 1] = hit_pos[3 * i + 1];
     pos[3 * k + 2] = hit_pos[3 * i + 2];
   }
-}
-
-__device__ inline uint32_t rgba8(float r, float g, float b) {
-  auto q = [](float v) { return (uint32_t)(fminf(fmaxf(v, 0.f), 1.f) * 255.f + 0.5f); };
-  return q(r) | (q(g) << 8) | (q(b) << 16) | (255u << 24);
 }
 
 // Background, then Lambert shading of the hits with the light at the eye

@@ -70,6 +70,27 @@ class mp_extract_job(ctypes.Structure):
     ]
 
 
+class mp_render_params(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("iters", ctypes.c_int * 3),
+        ("delta", ctypes.c_float),
+        ("threshold", ctypes.c_float),
+        ("normal_lod", ctypes.c_int),
+        ("shading", ctypes.c_int),
+        ("residual", ctypes.c_int),
+        ("skip_lod0", ctypes.c_int),
+        ("flip_y", ctypes.c_int),
+        ("lod2_reads_lod0", ctypes.c_int),
+        ("bound_radius", ctypes.c_float),
+        ("band_scale", ctypes.c_float),
+    ]
+
+
+MP_SHADE_NORMALS, MP_SHADE_PHONG = 0, 1
+
+
 class mp_trace_job(ctypes.Structure):
     _fields_ = [
         ("stack", ctypes.c_void_p),
@@ -105,6 +126,9 @@ SIGNATURES = {
     "mp_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                 c_int64_p, ctypes.c_void_p]),
+    "mp_render": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 c_int64_p, ctypes.c_void_p]),
     "mp_trace_batch_plan": (ctypes.c_int, [ctypes.POINTER(mp_trace_job), ctypes.c_int, ctypes.c_void_p,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "mp_trace_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

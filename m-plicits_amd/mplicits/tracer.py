@@ -1,4 +1,5 @@
-"""Multiscale sphere tracing on the GPU (mp_trace).
+"""Multiscale sphere tracing on the GPU (mp_trace) and the renderer's
+fixed-iteration frame (mp_render).
 
 The renderer-side counterpart of the reference: its CUDA renderer traces a
 level stack with fixed iteration counts per level on every pixel
@@ -73,6 +74,98 @@ def trace(stack, cameras, width, height, deltas=(), max_steps=128, hit_eps=1e-4,
     if normals:
         out["normals"] = nrm
     return out
+
+
+def _render_params(width, height, iters, delta, threshold, normal_lod, shading, residual, skip_lod0, flip_y,
+                   lod2_reads_lod0, bound_radius, band_scale):
+    if shading not in ("phong", "normals"):
+        raise ValueError(f"shading must be 'phong' or 'normals', got {shading!r}")
+    iters = tuple(int(i) for i in iters) + (0,) * (3 - len(iters))
+    prm = _lib.mp_render_params()
+    prm.width, prm.height = int(width), int(height)
+    for i in range(3):
+        prm.iters[i] = iters[i]
+    prm.delta, prm.threshold, prm.normal_lod = float(delta), float(threshold), int(normal_lod)
+    prm.shading = _lib.MP_SHADE_PHONG if shading == "phong" else _lib.MP_SHADE_NORMALS
+    prm.residual, prm.skip_lod0, prm.flip_y = int(bool(residual)), int(bool(skip_lod0)), int(bool(flip_y))
+    prm.lod2_reads_lod0 = int(bool(lod2_reads_lod0))
+    prm.bound_radius, prm.band_scale = float(bound_radius), float(band_scale)
+    return prm
+
+
+class FrameRenderer:
+    """The renderer's fixed-iteration frame (mp_render): ``iters`` steps per
+    level phase (level 0; levels 0 + 1; levels 0 (+ 1) + 2), Phong or normal
+    shading with the normal of levels 0..normal_lod, white background --
+    MIPplicit::SdfInference with the renderer's -iters / -delta / -threshold /
+    -normal_lod / -shading / -skip_lod0 / -no_residual options
+    (renderer/cuda/state.cu:816-1055, renderer/cuda/renderer.cu:754-790).
+
+    Outputs are preallocated GPU tensors: rgba [n, H, W] int32 (RGBA8), pos
+    [n, H, W, 3], dist [n, H, W], normals [n, H, W, 3].  ``frame(graph=True)``
+    runs the first frame eagerly, captures the next into a CUDA/HIP graph and
+    replays it afterwards (the reference captures its tracing loop the same
+    way, state.cu:850)."""
+
+    def __init__(self, stack, cameras, width, height, iters=(20, 0, 0), delta=0.0, threshold=0.05, normal_lod=0,
+                 shading="phong", residual=True, skip_lod0=False, flip_y=False, lod2_reads_lod0=False,
+                 bound_radius=1.2, band_scale=1.4):
+        if not isinstance(stack, LevelStack):
+            stack = LevelStack(stack, precision=getattr(stack[0], "precision", "fp32"))
+        stack.refresh()
+        self.stack = stack
+        self.lib = _lib.load()
+        cams = cameras if isinstance(cameras, torch.Tensor) else camera_block(cameras, stack.device)
+        if cams.dtype != torch.float32 or cams.dim() != 2 or cams.shape[1] != 32:
+            raise ValueError("cameras must be a [n, 32] float32 tensor or a list of (inv_view, inv_proj)")
+        self.cams = cams.to(stack.device).contiguous()
+        self.n = self.cams.shape[0]
+        self.prm = _render_params(width, height, iters, delta, threshold, normal_lod, shading, residual, skip_lod0,
+                                  flip_y, lod2_reads_lod0, bound_radius, band_scale)
+        dev = stack.device
+        self.rgba = torch.empty(self.n, height, width, dtype=torch.int32, device=dev)
+        self.pos = torch.empty(self.n, height, width, 3, dtype=torch.float32, device=dev)
+        self.dist = torch.empty(self.n, height, width, dtype=torch.float32, device=dev)
+        self.normals = torch.empty(self.n, height, width, 3, dtype=torch.float32, device=dev)
+        self._graph = None
+        self._ran = False
+
+    def _launch(self, stats=None):
+        _lib.check(self.lib.mp_render(self.stack._handle, _lib.ptr(self.cams), self.n, ctypes.byref(self.prm),
+                                      _lib.ptr(self.rgba), _lib.ptr(self.pos), _lib.ptr(self.dist),
+                                      _lib.ptr(self.normals), stats, _lib.stream_handle()))
+
+    def frame(self, graph=False):
+        """Render one frame into the output tensors (asynchronous)."""
+        with torch.cuda.device(self.stack.device):
+            if not graph or not self._ran:
+                self._launch()
+                self._ran = True
+                return
+            if self._graph is None:
+                torch.cuda.synchronize()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    self._launch()
+                self._graph = g
+            self._graph.replay()
+
+    def stats(self):
+        """Render one frame eagerly and return (level evaluations [3], shaded
+        pixels) (host-sync point)."""
+        st = (ctypes.c_int64 * 4)()
+        with torch.cuda.device(self.stack.device):
+            self._launch(ctypes.cast(st, _lib.c_int64_p))
+        return [int(st[i]) for i in range(3)], int(st[3])
+
+
+def render(stack, cameras, width, height, **kw):
+    """One fixed-iteration frame (see FrameRenderer).  Returns a dict of GPU
+    tensors rgba, pos, dist, normals and the host-side stats evals (level
+    evaluations performed) and shaded.  Synchronises the stream."""
+    r = FrameRenderer(stack, cameras, width, height, **kw)
+    evals, shaded = r.stats()
+    return {"rgba": r.rgba, "pos": r.pos, "dist": r.dist, "normals": r.normals, "evals": evals, "shaded": shaded}
 
 
 class BatchedTrace:
