@@ -34,8 +34,9 @@ from mplicits.meshing import marching_cubes_gpu  # noqa: E402
 from mplicits.model import from_pth  # noqa: E402
 from mplicits.stack import LevelStack  # noqa: E402
 from mplicits.batch import BatchedExtraction  # noqa: E402
-from mplicits.synthetic import DELTAS, MASTER_SEED, BumpySphere, orbit_cameras, residual_levels, uneven_workload  # noqa: E402
-from mplicits.tracer import BatchedTrace, camera_block, trace  # noqa: E402
+from mplicits.synthetic import (DELTAS, MASTER_SEED, BumpySphere, orbit_cameras, residual_levels,  # noqa: E402
+                                tab4_levels, uneven_workload)
+from mplicits.tracer import BatchedTrace, FrameRenderer, camera_block, renderer_camera, trace  # noqa: E402
 from mplicits import training  # noqa: E402
 
 N = 512
@@ -175,6 +176,46 @@ def bench_trace(levels, precisions, reps=2):
     return out
 
 
+# Rows of the renderer benchmark at 512^2 (reference reproduce.py:57-64): label,
+# levels, options, (paper, reproduced) FPS on an RTX 5090 in FP16.
+TAB4 = [
+    ("(400,4) SIREN baseline", ["siren"], dict(iters=(20, 0, 0)), (19, 16)),
+    ("(128,2) coarse", ["coarse"], dict(iters=(20, 0, 0)), (180, 118)),
+    ("(128,2)>(256,2) (NM)", ["coarse", "medium", "fine"], dict(iters=(20, 0, 0), delta=0.02, normal_lod=1), (85, 64)),
+    ("(128,2)>(256,2)", ["coarse", "medium", "fine"], dict(iters=(20, 5, 0), delta=0.02, normal_lod=1), (70, 62)),
+    ("(128,2)>(256,2)>(400,2) (NM)", ["coarse", "medium", "fine"], dict(iters=(20, 5, 0), delta=0.02, normal_lod=2),
+     (43, 35)),
+    ("(128,2)>(256,2)>(400,2)", ["coarse", "medium", "fine"], dict(iters=(20, 5, 5), delta=0.02, normal_lod=2),
+     (35, 31)),
+]
+
+
+def bench_render(precisions=("fp16", "fp32"), frames=100, warmup=30):
+    """The renderer benchmark's rows on synthetic levels of the same shapes:
+    512x512 fixed-iteration frames from the renderer's initial camera, replayed
+    from a captured graph after the renderer's 30 warm-up frames; FPS."""
+    lv = tab4_levels("cuda")
+    cams = camera_block([renderer_camera(512, 512)])
+    out = {}
+    for label, names, opts, ref in TAB4:
+        row = {"iters": "/".join(str(i) for i in opts["iters"]), "normal_lod": opts.get("normal_lod", 0),
+               "reference_fps_rtx5090": {"paper": ref[0], "reproduced": ref[1]}}
+        for prec in precisions:
+            r = FrameRenderer(LevelStack([lv[k] for k in names], prec), cams, 512, 512, **opts)
+            evals, shaded = r.stats()
+            for _ in range(warmup):
+                r.frame(graph=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                r.frame(graph=True)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / frames
+            row[prec] = {"fps": round(1.0 / dt, 1), "ms": round(dt * 1e3, 3), "shaded": shaded, "evals": evals}
+        out[label] = row
+    return out
+
+
 def bench_batch(precision="bf16", reps=3, n_models=24):
     """Batched uneven workload: 24 seeded models (width 64-512, 1-3 levels,
     own bumpy-sphere target, grids 128^3-512^3, one 1280x720 camera each).
@@ -304,6 +345,9 @@ def main():
     tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
     if tn:
         print(f"# train: {tn}", file=sys.stderr, flush=True)
+    rd = bench_render() if not args.no_extras else None
+    if rd:
+        print(f"# render: {rd}", file=sys.stderr, flush=True)
     bt = bench_batch() if not args.no_extras else None
     if bt:
         print(f"# batch: {bt}", file=sys.stderr, flush=True)
@@ -345,6 +389,7 @@ def main():
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
         "trace_16x1080p": tr,
         "train_step_2e18": tn,
+        "render_512_tab4": rd,
         "batch_24_uneven": bt,
         "roofline": roofline,
         "cpu_baseline": cpu,

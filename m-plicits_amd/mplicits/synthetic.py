@@ -193,6 +193,18 @@ def fit_output_layer(model, shape, n=20000, seed=0, ridge=1e-6):
     return model
 
 
+def tab4_levels(device="cpu", seed=MASTER_SEED + 600):
+    """Levels shaped like the rows of the renderer benchmark (width, hidden
+    layers): coarse (128, 2) and the single-SIREN baseline (400, 4), omega 30,
+    with output layers fitted to the bumpy sphere; residual medium (256, 2),
+    omega 60, and fine (400, 2), omega 90, seeded with last layers x 0.1."""
+    shape = BumpySphere()
+    return {"coarse": fit_output_layer(seeded_siren([128] * 2, 30.0, seed, device=device), shape, seed=seed),
+            "medium": seeded_siren([256] * 2, 60.0, seed + 1, last_scale=0.1, device=device),
+            "fine": seeded_siren([400] * 2, 90.0, seed + 2, last_scale=0.1, device=device),
+            "siren": fit_output_layer(seeded_siren([400] * 4, 30.0, seed + 3, device=device), shape, seed=seed + 3)}
+
+
 def uneven_workload(n_models=24, seed=MASTER_SEED + 500, device="cpu"):
     """Batched uneven workload: per model a width in {64, 128, 256, 512}, 1-3
     levels, its own bumpy-sphere target (sub-seed), a grid resolution in

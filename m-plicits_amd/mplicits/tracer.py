@@ -27,6 +27,15 @@ def camera_block(cameras, device="cuda"):
     return torch.tensor(np.stack(rows), dtype=torch.float32, device=device)
 
 
+def renderer_camera(width, height):
+    """(inv_view, inv_proj) of the renderer's initial arcball view:
+    translate(0, 0, -3.5), 45 degree field of view, near 1e-5, far 1e4
+    (renderer/cuda/camera.cu:9-17,77,131)."""
+    from .synthetic import look_at, perspective
+    return (np.linalg.inv(look_at((0.0, 0.0, 3.5))),
+            np.linalg.inv(perspective(45.0, width / height, near=1e-5, far=1e4)))
+
+
 def _params(width, height, deltas, max_steps, hit_eps, bound_radius, band_scale, pixel_center, flip_y):
     prm = _lib.mp_trace_params()
     prm.width, prm.height, prm.max_steps = int(width), int(height), int(max_steps)

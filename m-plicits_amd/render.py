@@ -1,15 +1,20 @@
 #!/usr/bin/env python
-"""Headless counterpart of the renderer executable's model loading and
--benchmark protocol (renderer/cuda/renderer.cu:568-595, state.cu:662-712):
-load an .exp descriptor and its .bin checkpoints, sphere-trace the three-level
-stack on the GPU, write the last frame as a PPM and print the BENCHMARK line.
+"""Headless counterpart of the renderer executable (renderer/cuda/renderer.cu):
+its model loading (.exp descriptor and .bin checkpoints, state.cu:662-712),
+its tracing options (-iters, -delta, -threshold, -normal_lod, -shading,
+-skip_lod0, -no_residual; renderer.cu:754-790) and its -benchmark protocol
+(renderer.cu:568-595: 30 warm-up frames, then N timed frames, the BENCHMARK
+line, a line appended to benchmark.csv and the last frame written as a PPM).
 
     python m-plicits_amd/render.py -experiment_file=data/released/model.exp \\
-        -data_root=. -resolution=512x512 -benchmark=100 -delta=0.02
+        -data_root=. -benchmark=500 -iters=20,5,5 -delta=0.02 -normal_lod=2
 
-Tracing runs until every ray hits (|F| < -eps) or leaves the bounding sphere
-(at most -max_steps steps) instead of the renderer's fixed per-level
-iteration counts; the BENCHMARK line reports that as iters=adaptive:<max>.
+The default camera is the renderer's initial arcball view (eye at (0, 0, 3.5)
+looking down -z, 45 degree field of view).  Frames are the renderer's
+fixed-iteration frames (mp_render), replayed from a captured graph as the
+renderer replays its CUDA graph.  -adaptive traces with the persistent tracer
+instead (mp_trace: per-ray hit test, at most -max_steps steps); its BENCHMARK
+line reports iters=adaptive:<max_steps>.
 """
 import argparse
 import os.path as osp
@@ -22,8 +27,9 @@ import torch
 sys.path.insert(0, osp.dirname(osp.abspath(__file__)))
 from mplicits.renderer_io import load_experiment  # noqa: E402
 from mplicits.stack import LevelStack  # noqa: E402
-from mplicits.synthetic import look_at, perspective  # noqa: E402
-from mplicits.tracer import camera_block, trace  # noqa: E402
+from mplicits.tracer import FrameRenderer, camera_block, renderer_camera, trace  # noqa: E402
+
+WARMUP_FRAMES = 30  # g_bench_warmup, renderer.cu:82
 
 
 def _args(argv):
@@ -37,45 +43,89 @@ def _args(argv):
     ap.add_argument("--experiment_file", required=True)
     ap.add_argument("--data_root", default=None)
     ap.add_argument("--resolution", default="512x512")
-    ap.add_argument("--benchmark", type=int, default=0, help="frames to time (0: render one frame)")
-    ap.add_argument("--delta", type=float, default=0.02, help="band width of the coarse level")
-    ap.add_argument("--delta1", type=float, default=None, help="band width of the medium level (default: delta)")
+    ap.add_argument("--benchmark", type=int, default=None, help="frames to time (<= 0: 500; absent: one frame)")
+    ap.add_argument("--iters", default="20,0,0", help="sphere-tracing steps of the level 0 / 1 / 2 phases")
+    ap.add_argument("--delta", default="0", help="band of level 0 (and of level 1: d0,d1)")
+    ap.add_argument("--threshold", type=float, default=0.05)
+    ap.add_argument("--normal_lod", type=int, default=0)
+    ap.add_argument("--shading", default="phong", choices=["phong", "normals"])
+    ap.add_argument("--skip_lod0", action="store_true")
+    ap.add_argument("--no_residual", action="store_true")
+    ap.add_argument("--lod2_reads_lod0", action="store_true",
+                    help="level 2's distances read level 0's output, as the renderer binds them")
+    ap.add_argument("--adaptive", action="store_true", help="persistent tracer with a per-ray hit test")
     ap.add_argument("--max_steps", type=int, default=128)
     ap.add_argument("--eps", type=float, default=1e-4)
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "bf16", "fp16"],
+                    help="the renderer computes in FP16")
+    ap.add_argument("--csv", default="benchmark.csv")
     ap.add_argument("--out", default="img_render.ppm")
     return ap.parse_args(norm)
+
+
+def write_ppm(path, rgba, flip_y):
+    v = rgba.cpu().numpy().astype(np.int64) & 0xFFFFFFFF
+    rgb = np.stack([(v >> s) & 255 for s in (0, 8, 16)], -1).astype(np.uint8)
+    h, w = rgb.shape[:2]
+    with open(path, "wb") as f:
+        f.write(f"P6 {w} {h} 255\n".encode())
+        f.write(rgb[::-1].tobytes() if not flip_y else rgb.tobytes())
 
 
 def main(argv=None):
     args = _args(sys.argv[1:] if argv is None else argv)
     if not torch.cuda.is_available():
-        raise SystemExit("render.py needs a GPU (the tracer runs only through the HIP library)")
+        raise SystemExit("render.py needs a GPU (frames are drawn only through the HIP library)")
     width, height = (int(v) for v in args.resolution.lower().split("x"))
+    iters = [int(v) for v in args.iters.split(",")] + [0, 0]
+    iters = iters[:3]
+    deltas = [float(v) for v in args.delta.split(",")]
+    d0, d1 = deltas[0], (deltas[1] if len(deltas) > 1 else deltas[0])
     levels, desc = load_experiment(args.experiment_file, data_root=args.data_root, device="cuda")
+    flip_y = desc.get("flip_y", False)
     stack = LevelStack(levels, args.precision)
-    deltas = [args.delta, args.delta if args.delta1 is None else args.delta1]
-    eye = np.array([0.0, 0.0, 2.5])
-    cams = camera_block([(np.linalg.inv(look_at(eye)), np.linalg.inv(perspective(45.0, width / height)))])
-    kw = dict(deltas=deltas, max_steps=args.max_steps, hit_eps=args.eps, flip_y=desc.get("flip_y", False))
-    out = trace(stack, cams, width, height, **kw)
-    if args.benchmark > 0:
-        for _ in range(3):
-            trace(stack, cams, width, height, **kw)
+    cams = camera_block([renderer_camera(width, height)])
+    frames = None if args.benchmark is None else (args.benchmark if args.benchmark > 0 else 500)
+
+    if args.adaptive:
+        kw = dict(deltas=[d0, d1][:stack.n_levels - 1], max_steps=args.max_steps, hit_eps=args.eps, flip_y=flip_y,
+                  pixel_center=False)
+
+        def draw():
+            return trace(stack, cams, width, height, **kw)
+        out = draw()
+        rgba = out["rgba"][0]
+        iters_txt = f"adaptive:{args.max_steps}"
+        info = f"hits {out['hits']}, evaluations per level {out['evals']}"
+    else:
+        r = FrameRenderer(stack, cams, width, height, iters=iters, delta=d0, threshold=args.threshold,
+                          normal_lod=args.normal_lod, shading=args.shading, residual=not args.no_residual,
+                          skip_lod0=args.skip_lod0, flip_y=flip_y, lod2_reads_lod0=args.lod2_reads_lod0)
+        evals, shaded = r.stats()
+
+        def draw():
+            r.frame(graph=True)
+        rgba = r.rgba[0]
+        iters_txt = "/".join(str(i) for i in iters)
+        info = f"shaded {shaded}, evaluations per level {evals}"
+    if frames:
+        for _ in range(WARMUP_FRAMES):
+            draw()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.benchmark):
-            out = trace(stack, cams, width, height, **kw)
+        for _ in range(frames):
+            out = draw()
         torch.cuda.synchronize()
-        fps = args.benchmark / (time.perf_counter() - t0)
-        print(f"BENCHMARK resolution={width}x{height} iters=adaptive:{args.max_steps} delta={args.delta:.4f} "
-              f"normal_lod=2 frames={args.benchmark} avg_fps={fps:.1f}")
-    rgba = out["rgba"][0].cpu().numpy().astype(np.uint32)
-    rgb = np.stack([(rgba >> s) & 255 for s in (0, 8, 16)], -1).astype(np.uint8)
-    with open(args.out, "wb") as f:
-        f.write(f"P6 {width} {height} 255\n".encode())
-        f.write(rgb[::-1].tobytes() if not desc.get("flip_y", False) else rgb.tobytes())
-    print(f"hits {out['hits']}, evaluations per level {out['evals']}, image {args.out}")
+        fps = frames / (time.perf_counter() - t0)
+        if args.adaptive:
+            rgba = out["rgba"][0]
+        print(f"BENCHMARK resolution={width}x{height} iters={iters_txt} delta={d0:.4f} "
+              f"normal_lod={args.normal_lod} frames={frames} avg_fps={fps:.1f}")
+        if args.csv:
+            with open(args.csv, "a", encoding="utf-8") as f:
+                f.write(f"{width}x{height},{iters_txt},{d0:.4f},{args.normal_lod},{frames},{fps:.2f}\n")
+    write_ppm(args.out, rgba, flip_y)
+    print(f"{info}, image {args.out}")
 
 
 if __name__ == "__main__":

@@ -82,7 +82,9 @@ def test_stack_matches_oracle_step_rule():
     assert out["evals"][0] > 0 and out["evals"][1] > 0
 
 
-def test_render_cli_benchmark_line(tmp_path, capsys):
+@pytest.mark.parametrize("flags,iters", [(["-iters=20,5,5", "-delta=0.05", "-normal_lod=2"], "20/5/5"),
+                                         (["-adaptive", "-delta=0.05"], "adaptive:128")])
+def test_render_cli_benchmark_line(tmp_path, capsys, flags, iters):
     import importlib.util
     import os.path as osp
     from mplicits import renderer_io
@@ -95,7 +97,8 @@ def test_render_cli_benchmark_line(tmp_path, capsys):
     render = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(render)
     render.main([f"-experiment_file={desc}", f"-data_root={tmp_path}", "-resolution=64x48", "-benchmark=2",
-                 f"-out={tmp_path / 'img.ppm'}"])
+                 f"-out={tmp_path / 'img.ppm'}", f"-csv={tmp_path / 'benchmark.csv'}"] + flags)
     out = capsys.readouterr().out
-    assert "BENCHMARK resolution=64x48" in out and "frames=2 avg_fps=" in out
+    assert f"BENCHMARK resolution=64x48 iters={iters} delta=0.0500" in out and "frames=2 avg_fps=" in out
     assert open(tmp_path / "img.ppm", "rb").read(11) == b"P6 64 48 25"
+    assert open(tmp_path / "benchmark.csv").read().startswith(f"64x48,{iters},0.0500,")
