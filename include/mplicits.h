@@ -298,6 +298,40 @@ int mp_train_first_bwd(const float* Z, const float* Pbar, const float* x, const 
 int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
                      int n_tangents, float* Pbar, float* dwl_part, void* stream);
 
+/* Training-data generation (the step before the training kernels).
+ * mp_mesh: a triangle mesh for unsigned point-to-mesh distances, replacing
+ * open3d's RaycastingScene (add_triangles / compute_distance) as the trainers
+ * use it (experiment_scripts/train_sdf_on_neighborhood_fine.py:145-146,
+ * experiment_scripts/utils.py:77-78).  verts [n_verts][3] float32 and faces
+ * [n_faces][3] int32 are HOST arrays; the bounding-volume hierarchy is built
+ * on the host and uploaded (host-sync point, allocates). */
+typedef struct mp_mesh mp_mesh;
+int mp_mesh_create(const float* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces, void* stream,
+                   mp_mesh** out);
+void mp_mesh_destroy(mp_mesh* mesh);
+int mp_mesh_info(const mp_mesh* mesh, int64_t* n_tris, int64_t* n_nodes, int* depth);
+/* dist [n] = unsigned distance of device points pts [n][3] to the mesh. */
+int mp_mesh_distance(const mp_mesh* mesh, const float* pts, int64_t n, float* dist, void* stream);
+/* calc_deltas (experiment_scripts/utils.py:46-89): vert [V][cols] device
+ * (x, y, z, nx, ny, nz, ...); out [V][7] = (x, y, z, unit normal, delta)
+ * where delta is searched along the normal (start max_delta / n_iters, at
+ * most n_iters steps while max(delta) < max_delta, eps 2e-4) for the offset
+ * whose distance to the mesh equals it.  iters_done (host, may be NULL):
+ * iterations run.  Host-sync point (one per iteration, as the reference's
+ * loop test), allocates scratch. */
+int mp_calc_deltas(const mp_mesh* mesh, const float* vert, int64_t V, int cols, float max_delta, int n_iters,
+                   float* out, int* iters_done, void* stream);
+/* create_training_data_on_neighborhood (experiment_scripts/utils.py:92-169)
+ * given its random draws: idx [n] int64 sampled vertex rows (NULL: rows 0..n-1,
+ * the reference's n == V case), rnd [n][3] its uniform numbers.  Writes
+ * pts / normals [m][3] and sdfs [m], m = 3n with dithering, else 2n: the
+ * vertices (sdf 0), offsets along the normal by delta_n * rnd_z (sdf the same;
+ * delta_n = the vertex's last column if use_neigh_delta, else delta_eps) and
+ * the dithered band points p + delta / 2 n + delta rnd (sdf -1). */
+int mp_neighborhood_samples(const float* vert, int64_t V, int cols, const int64_t* idx, const float* rnd, int64_t n,
+                            float delta, float delta_eps, int use_neigh_delta, int dithering, float* pts,
+                            float* normals, float* sdfs, void* stream);
+
 /* Marching cubes of a [nz][ny][nx] float32 device volume at `level`: replaces
  * skimage.measure.marching_cubes(volume, level) as convert_sdf_samples_to_ply
  * calls it (i3d/meshing.py:296-298; Lewiner method, 'descent' winding,

@@ -165,6 +165,19 @@ SIGNATURES = {
     "mp_train_out_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_mesh_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "mp_mesh_destroy": (None, [ctypes.c_void_p]),
+    "mp_mesh_info": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p, ctypes.POINTER(ctypes.c_int)]),
+    "mp_mesh_distance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "mp_calc_deltas": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                      ctypes.c_float, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
+    "mp_neighborhood_samples": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
     "mp_extract_batch_plan": (ctypes.c_int, [ctypes.POINTER(mp_extract_job), ctypes.c_int, ctypes.c_void_p,
                                              ctypes.POINTER(ctypes.c_void_p)]),
     "mp_extract_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
