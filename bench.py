@@ -176,6 +176,60 @@ def bench_trace(levels, precisions, reps=2):
     return out
 
 
+def bench_train_data(levels, n_queries=1 << 20, n_on=1 << 17, iters=100):
+    """Training-data generation on the coarse level's 512^3 surface (GPU
+    marching cubes, vertices scaled to [-1, 1]^3 with the MC normals): host
+    hierarchy build, 2^20 random distance queries, calc_deltas over every
+    vertex (max_delta 0.05, 100 iterations) and one neighbourhood sample of
+    2^17 surface points (3 x 2^17 training points)."""
+    from mplicits import sampling
+    st = LevelStack(levels[:1], "bf16")
+    vol = torch.empty(N ** 3, device="cuda")
+    st.extract_volume(N, [], chunk=CHUNK, out=vol)
+    v, f, nrm, _ = marching_cubes_gpu(vol.view(N, N, N), 0.0)
+    v = v / (N - 1) * 2 - 1
+    verts = torch.cat([v, nrm], 1).contiguous()
+    del vol
+    t0 = time.perf_counter()
+    scene = sampling.RaycastingScene()
+    scene.add_triangles(v.cpu().numpy(), f.cpu().numpy())
+    t_build = time.perf_counter() - t0
+    tris, nodes, depth = scene.info()
+    q = torch.rand(n_queries, 3, device="cuda") * 2 - 1
+    scene.compute_distance(q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        scene.compute_distance(q)
+    torch.cuda.synchronize()
+    t_q = (time.perf_counter() - t0) / 3
+    near = verts[:n_queries, :3] + 0.01 * verts[:n_queries, 3:6]  # points near the surface
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    scene.compute_distance(near)
+    torch.cuda.synchronize()
+    t_near = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = sampling.calc_deltas(verts, scene, max_delta=0.05, n_iters=iters, device="cuda")
+    torch.cuda.synchronize()
+    t_cd = time.perf_counter() - t0
+    sampling.create_training_data_on_neighborhood(out, n_on, device="cuda", delta=0.065, use_neigh_delta=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        sampling.create_training_data_on_neighborhood(out, n_on, device="cuda", delta=0.065, use_neigh_delta=True)
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / 10
+    scene.close()
+    return {"mesh": {"verts": int(v.shape[0]), "tris": tris, "bvh_nodes": nodes, "bvh_depth": depth},
+            "bvh_build_s": round(t_build, 3),
+            "distance_random_mqueries_per_s": round(n_queries / t_q / 1e6, 1),
+            "distance_near_surface_mqueries_per_s": round(near.shape[0] / t_near / 1e6, 1),
+            "calc_deltas_ms": round(t_cd * 1e3, 1), "calc_deltas_iterations": sampling.calc_deltas.iterations,
+            "sample_ms": round(t_s * 1e3, 3), "sample_points": 3 * n_on}
+
+
 # Rows of the renderer benchmark at 512^2 (reference reproduce.py:57-64): label,
 # levels, options, (paper, reproduced) FPS on an RTX 5090 in FP16.
 TAB4 = [
@@ -345,6 +399,9 @@ def main():
     tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
     if tn:
         print(f"# train: {tn}", file=sys.stderr, flush=True)
+    td = bench_train_data(levels) if not args.no_extras else None
+    if td:
+        print(f"# train data: {td}", file=sys.stderr, flush=True)
     rd = bench_render() if not args.no_extras else None
     if rd:
         print(f"# render: {rd}", file=sys.stderr, flush=True)
@@ -389,6 +446,7 @@ def main():
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
         "trace_16x1080p": tr,
         "train_step_2e18": tn,
+        "train_data_512_mesh": td,
         "render_512_tab4": rd,
         "batch_24_uneven": bt,
         "roofline": roofline,
