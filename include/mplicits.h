@@ -285,16 +285,17 @@ int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const f
                          int n_tangents, void* Abar, int a_dtype, void* Ab, float* part, void* stream);
 /* mp_train_act_bwd with the bias gradient as per-block partial sums
  * db_part [mp_train_out_bwd_blocks(n)][W] instead of the [n][W] value rows
- * (W a multiple of 4, at most 512). */
-int mp_train_act_bwd_part(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
-                          int a_dtype, void* Ab, float* db_part, void* stream);
+ * (W a multiple of 4, at most 512).  p_dtype: element type of Pbar (0 float32,
+ * 1 bfloat16, 2 float16; also for mp_train_first_bwd). */
+int mp_train_act_bwd_part(const float* Z, const void* Pbar, int p_dtype, int64_t n, int W, int n_tangents,
+                          void* Abar, int a_dtype, void* Ab, float* db_part, void* stream);
 /* Layer-0 backward fused with its parameter gradients (W a multiple of 4,
  * at most 512): part [mp_train_first_bwd_blocks(n)][W][4] receives per-block
  * partial sums of (dW_0[u][0..2], db_0[u]) = sum over points of
  * abar x + sum_t adbar_t v_t and abar, with abar / adbar as in mp_train_act_bwd. */
 int64_t mp_train_first_bwd_blocks(int64_t n);
-int mp_train_first_bwd(const float* Z, const float* Pbar, const float* x, const float* v, int64_t n, int W,
-                       int n_tangents, float* part, void* stream);
+int mp_train_first_bwd(const float* Z, const void* Pbar, int p_dtype, const float* x, const float* v, int64_t n,
+                       int W, int n_tangents, float* part, void* stream);
 int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
                      int n_tangents, float* Pbar, float* dwl_part, void* stream);
 

@@ -121,6 +121,14 @@ __device__ inline void st4(T* p, int64_t i, float4 v) {
 template <>
 __device__ inline void st4<float>(float* p, int64_t i, float4 v) { *reinterpret_cast<float4*>(p + i) = v; }
 __device__ inline float4 ld4(const float* p, int64_t i) { return *reinterpret_cast<const float4*>(p + i); }
+__device__ inline float4 ld4(const __bf16* p, int64_t i) {
+  const Vec4<__bf16>::type v = *reinterpret_cast<const Vec4<__bf16>::type*>(p + i);
+  return make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
+__device__ inline float4 ld4(const _Float16* p, int64_t i) {
+  const Vec4<_Float16>::type v = *reinterpret_cast<const Vec4<_Float16>::type*>(p + i);
+  return make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
 
 // K-blocked copy of a [n][4][W] operand block (three tangents), the layout of
 // the weight-gradient GEMM: element (row r = 4 p + t, unit u) at
@@ -584,10 +592,10 @@ __global__ __launch_bounds__(128) void k_train_out_act_bwd4(const float* __restr
 
 // Hidden-layer backward through sin / the tangent products (as
 // k_train_act_bwd4) with the bias gradient reduced per block: blockDim =
-// W / 4, db_part [blocks][W].
-template <class T, int NT>
+// W / 4, db_part [blocks][W].  P: element type of Pbar.
+template <class T, class P, int NT>
 __global__ __launch_bounds__(128) void k_train_act_bwd_part4(const float* __restrict__ Z,
-                                                             const float* __restrict__ Pbar, int64_t n, int W,
+                                                             const P* __restrict__ Pbar, int64_t n, int W,
                                                              T* __restrict__ Abar, T* __restrict__ Ab,
                                                              float* __restrict__ db_part) {
   const int u = threadIdx.x * 4;
@@ -643,8 +651,8 @@ __global__ __launch_bounds__(128) void k_train_act_bwd_part4(const float* __rest
 // adbar_t = cos a pdbar_t are reduced in place against the layer input
 // [x; v_t], giving per-block partial sums part [blocks][W][4] of
 // (dW0[u][0..2], db0[u]).
-template <int NT>
-__global__ __launch_bounds__(128) void k_train_first_bwd4(const float* __restrict__ Z, const float* __restrict__ Pbar,
+template <class P, int NT>
+__global__ __launch_bounds__(128) void k_train_first_bwd4(const float* __restrict__ Z, const P* __restrict__ Pbar,
                                                           const float* __restrict__ x, const float* __restrict__ v,
                                                           int64_t n, int W, float* __restrict__ part) {
   const int u = threadIdx.x * 4;
@@ -775,11 +783,35 @@ int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const f
   return MP_OK;
 }
 
-int mp_train_act_bwd_part(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
-                          int a_dtype, void* Ab, float* db_part, void* stream) {
+}  // extern "C"
+
+namespace mp {
+
+template <class T, class P>
+static void launch_act_bwd_part(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
+                                int64_t n, void* Abar, void* Ab, float* db_part) {
+  const P* pb = reinterpret_cast<const P*>(Pbar);
+  T* a = reinterpret_cast<T*>(Abar);
+  T* ab = reinterpret_cast<T*>(Ab);
+  const dim3 g((unsigned)blocks), b(W / 4);
+  switch (nt) {
+    case 0: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 0>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    case 1: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 1>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    case 2: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 2>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    default: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 3>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+  }
+}
+
+}  // namespace mp
+
+extern "C" {
+
+int mp_train_act_bwd_part(const float* Z, const void* Pbar, int p_dtype, int64_t n, int W, int n_tangents,
+                          void* Abar, int a_dtype, void* Ab, float* db_part, void* stream) {
   MP_REQUIRE(Z && Pbar && Abar && db_part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
              "mp_train_act_bwd_part: bad arguments");
-  MP_REQUIRE(a_dtype >= 0 && a_dtype <= 2, MP_EINVAL, "mp_train_act_bwd_part: a_dtype %d", a_dtype);
+  MP_REQUIRE(a_dtype >= 0 && a_dtype <= 2 && p_dtype >= 0 && p_dtype <= 2, MP_EINVAL,
+             "mp_train_act_bwd_part: a_dtype %d, p_dtype %d", a_dtype, p_dtype);
   MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED,
              "mp_train_act_bwd_part: width %d (needs a multiple of 4, <= 512)", W);
   MP_REQUIRE(!Ab || (n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
@@ -787,36 +819,56 @@ int mp_train_act_bwd_part(const float* Z, const float* Pbar, int64_t n, int W, i
   if (n == 0) return MP_OK;
   const int64_t blocks = mp_train_out_bwd_blocks(n);
   hipStream_t st = as_stream(stream);
-  if (a_dtype == 1)
-    MP_TRAIN_RED_DISPATCH(k_train_act_bwd_part4, __bf16, n_tangents, blocks, W, st, Z, Pbar, n, W,
-                          reinterpret_cast<__bf16*>(Abar), reinterpret_cast<__bf16*>(Ab), db_part);
-  else if (a_dtype == 2)
-    MP_TRAIN_RED_DISPATCH(k_train_act_bwd_part4, _Float16, n_tangents, blocks, W, st, Z, Pbar, n, W,
-                          reinterpret_cast<_Float16*>(Abar), reinterpret_cast<_Float16*>(Ab), db_part);
-  else
-    MP_TRAIN_RED_DISPATCH(k_train_act_bwd_part4, float, n_tangents, blocks, W, st, Z, Pbar, n, W,
-                          reinterpret_cast<float*>(Abar), static_cast<float*>(nullptr), db_part);
+#define MP_ACT_BWD_P(T)                                                                                     \
+  do {                                                                                                    \
+    if (p_dtype == 1) launch_act_bwd_part<T, __bf16>(n_tangents, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part);    \
+    else if (p_dtype == 2) launch_act_bwd_part<T, _Float16>(n_tangents, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part); \
+    else launch_act_bwd_part<T, float>(n_tangents, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part);          \
+  } while (0)
+  if (a_dtype == 1) MP_ACT_BWD_P(__bf16);
+  else if (a_dtype == 2) MP_ACT_BWD_P(_Float16);
+  else MP_ACT_BWD_P(float);
+#undef MP_ACT_BWD_P
   MP_LAUNCHED("k_train_act_bwd_part4");
   return MP_OK;
 }
 
 int64_t mp_train_first_bwd_blocks(int64_t n) { return (n + RED_ROWS - 1) / RED_ROWS; }
 
-int mp_train_first_bwd(const float* Z, const float* Pbar, const float* x, const float* v, int64_t n, int W,
-                       int n_tangents, float* part, void* stream) {
+}  // extern "C"
+
+namespace mp {
+
+template <class P>
+static void launch_first_bwd(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
+                             const float* x, const float* v, int64_t n, float* part) {
+  const P* pb = reinterpret_cast<const P*>(Pbar);
+  const dim3 g((unsigned)blocks), b(W / 4);
+  switch (nt) {
+    case 0: hipLaunchKernelGGL((k_train_first_bwd4<P, 0>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
+    case 1: hipLaunchKernelGGL((k_train_first_bwd4<P, 1>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
+    case 2: hipLaunchKernelGGL((k_train_first_bwd4<P, 2>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
+    default: hipLaunchKernelGGL((k_train_first_bwd4<P, 3>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
+  }
+}
+
+}  // namespace mp
+
+extern "C" {
+
+int mp_train_first_bwd(const float* Z, const void* Pbar, int p_dtype, const float* x, const float* v, int64_t n,
+                       int W, int n_tangents, float* part, void* stream) {
   MP_REQUIRE(Z && Pbar && x && part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
              "mp_train_first_bwd: bad arguments");
+  MP_REQUIRE(p_dtype >= 0 && p_dtype <= 2, MP_EINVAL, "mp_train_first_bwd: p_dtype %d", p_dtype);
   MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first_bwd: tangent directions are NULL");
   MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED, "mp_train_first_bwd: width %d (needs a multiple of 4, <= 512)", W);
   if (n == 0) return MP_OK;
   const int64_t blocks = mp_train_first_bwd_blocks(n);
   hipStream_t st = as_stream(stream);
-  switch (n_tangents) {
-    case 0: hipLaunchKernelGGL(k_train_first_bwd4<0>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
-    case 1: hipLaunchKernelGGL(k_train_first_bwd4<1>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
-    case 2: hipLaunchKernelGGL(k_train_first_bwd4<2>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
-    default: hipLaunchKernelGGL(k_train_first_bwd4<3>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, Pbar, x, v, n, W, part); break;
-  }
+  if (p_dtype == 1) launch_first_bwd<__bf16>(n_tangents, blocks, W, st, Z, Pbar, x, v, n, part);
+  else if (p_dtype == 2) launch_first_bwd<_Float16>(n_tangents, blocks, W, st, Z, Pbar, x, v, n, part);
+  else launch_first_bwd<float>(n_tangents, blocks, W, st, Z, Pbar, x, v, n, part);
   MP_LAUNCHED("k_train_first_bwd4");
   return MP_OK;
 }

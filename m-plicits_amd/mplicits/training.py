@@ -164,7 +164,14 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
         else:
             dW = _mm(A2.t(), Xs[j - 1].view(-1, win), precision)
         grads[j] = (dW, db)
+        if precision != "fp32" and win % 4 == 0 and win <= 512:
+            # 16-bit seeds: their consumers round to 16 bit right after, and
+            # the GEMM writes half the bytes
+            return torch.mm(A2, _gemm_weight(Wj, precision)).view(n, 1 + nt, win)
         return _mm(A2, _gemm_weight(Wj, precision), precision).view(n, 1 + nt, win)
+
+    def pcode(P):
+        return {torch.float32: 0, torch.bfloat16: 1, torch.float16: 2}[P.dtype]
 
     if L >= 3 and wlast % 4 == 0 and wlast <= 512:
         # output layer and last sine layer in one pass: the rank-one seeds
@@ -190,11 +197,12 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
         A, Ab = adjoint(j)
         if wout % 4 == 0 and wout <= 512:
             dbp = torch.empty(nblk, wout, device=x.device, dtype=torch.float32)
-            _lib.check(lib.mp_train_act_bwd_part(_lib.ptr(Zs[j]), _lib.ptr(Pbar), n, wout, nt, _lib.ptr(A),
-                                                 _CODE[precision], _lib.ptr(Ab), _lib.ptr(dbp), stream))
+            _lib.check(lib.mp_train_act_bwd_part(_lib.ptr(Zs[j]), _lib.ptr(Pbar), pcode(Pbar), n, wout, nt,
+                                                 _lib.ptr(A), _CODE[precision], _lib.ptr(Ab), _lib.ptr(dbp), stream))
             db = dbp.sum(0)
         else:
             aval = torch.empty(n, wout, device=x.device, dtype=torch.float32)
+            Pbar = Pbar.float()
             _lib.check(lib.mp_train_act_bwd(_lib.ptr(Zs[j]), _lib.ptr(Pbar), n, wout, nt, _lib.ptr(A),
                                             _CODE[precision], _lib.ptr(aval), _lib.ptr(Ab), stream))
             db = aval.sum(0)
@@ -203,7 +211,7 @@ def backward_tangents(layers, x, V, cache, s=None, g=None, precision="fp32"):
     if w0 % 4 == 0 and w0 <= 512:
         # layer-0 adjoints reduced against [x; v_t] in the same pass
         part = torch.empty(int(lib.mp_train_first_bwd_blocks(n)), w0, 4, device=x.device, dtype=torch.float32)
-        _lib.check(lib.mp_train_first_bwd(_lib.ptr(Zs[0]), _lib.ptr(Pbar), _lib.ptr(x),
+        _lib.check(lib.mp_train_first_bwd(_lib.ptr(Zs[0]), _lib.ptr(Pbar), pcode(Pbar), _lib.ptr(x),
                                           _lib.ptr(V.contiguous() if nt else None), n, w0, nt, _lib.ptr(part),
                                           stream))
         tot = part.sum(0)
