@@ -364,6 +364,8 @@ def main():
                     help="comma list of paths to time (profiling runs use one)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB tensor (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--only", choices=["trace", "train", "train_data", "render", "batch"], default=None,
+                    help="run one of the side figures alone and print it as JSON (profiling runs)")
     args = ap.parse_args()
     if args.calibrate_copy:
         a = torch.empty(1 << 28, device="cuda")
@@ -380,6 +382,15 @@ def main():
         sys.exit(2)
     torch.cuda.set_device(0)
     levels = load_levels()
+    if args.only:
+        precs = args.precisions.split(",") if args.precisions != "bf16,fp16,fp32" else None
+        fig = {"trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"]),
+               "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps),
+               "train_data": lambda: bench_train_data(levels),
+               "render": lambda: bench_render(tuple(precs or ("fp16", "fp32")), frames=args.steps * 20),
+               "batch": lambda: bench_batch()}[args.only]()
+        print(json.dumps({args.only: fig}))
+        return
     vol = torch.empty(N, N, N, device="cuda")
 
     results = {}
