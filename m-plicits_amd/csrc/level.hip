@@ -7,30 +7,31 @@
 
 namespace mp {
 
-template <class S, int W, int NC>
-__global__ __launch_bounds__(256) void k_level(LevelDev L, EvalTarget t) {
+template <class S, int W, int NC, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_level(LevelDev L, EvalTarget t) {
   extern __shared__ uint4 lds_raw[];
   const int64_t n = (t.src == SRC_LIST) ? (int64_t)*t.list_count : t.n;
   // Workgroup-uniform exit: the waves of a workgroup may share barriers.
-  if ((int64_t)blockIdx.x * 4 * LevelCore<S, W, NC, true>::TILE >= n) return;
-  eval_block<S, W, NC>(L, t, n, blockIdx.x, reinterpret_cast<typename S::Frag*>(lds_raw));
+  if ((int64_t)blockIdx.x * WAVES * LevelCore<S, W, NC, true, WAVES>::TILE >= n) return;
+  eval_block<S, W, NC, WAVES>(L, t, n, blockIdx.x, reinterpret_cast<typename S::Frag*>(lds_raw));
 }
 
-template <class S, int W, int NC>
+template <class S, int W, int NC, int WAVES = 4>
 static int launch_typed(const LevelDev& L, const EvalTarget& t, hipStream_t stream) {
-  typedef LevelCore<S, W, NC, true> M;
-  const int64_t per_block = (int64_t)M::TILE * 4;
+  typedef LevelCore<S, W, NC, true, WAVES> M;
+  const int64_t per_block = (int64_t)M::TILE * WAVES;
   const int64_t blocks = (t.n + per_block - 1) / per_block;
   if (blocks <= 0) return MP_OK;
   MP_REQUIRE(blocks < (1ll << 31), MP_EINVAL, "level launch: %lld points is too many", (long long)t.n);
   static_assert(M::LDS_BYTES <= 160 * 1024, "LDS budget of one workgroup");
   static bool attr_set = false;
   if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level<S, W, NC>),
+    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level<S, W, NC, WAVES>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)M::LDS_BYTES));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_level<S, W, NC>), dim3((unsigned)blocks), dim3(256), M::LDS_BYTES, stream, L, t);
+  hipLaunchKernelGGL((k_level<S, W, NC, WAVES>), dim3((unsigned)blocks), dim3(64 * WAVES), M::LDS_BYTES, stream, L,
+                     t);
   MP_LAUNCHED("k_level");
   return MP_OK;
 }
@@ -40,10 +41,13 @@ static int launch_typed(const LevelDev& L, const EvalTarget& t, hipStream_t stre
 template <class S>
 static int launch_width(const LevelDev& L, int nc, const EvalTarget& t, hipStream_t stream) {
   if (nc == 1) {
+    // 16-bit single-column tiles up to width 256: two waves per SIMD (their
+    // registers and hand-off regions fit twice)
+    constexpr int W8 = S::ROWS == 32 ? 8 : 4;
     switch (L.width) {
-      case 64: return launch_typed<S, 64, 1>(L, t, stream);
-      case 128: return launch_typed<S, 128, 1>(L, t, stream);
-      case 256: return launch_typed<S, 256, 1>(L, t, stream);
+      case 64: return launch_typed<S, 64, 1, W8>(L, t, stream);
+      case 128: return launch_typed<S, 128, 1, W8>(L, t, stream);
+      case 256: return launch_typed<S, 256, 1, W8>(L, t, stream);
       case 512: return launch_typed<S, 512, 1>(L, t, stream);
     }
   } else {

@@ -35,7 +35,10 @@ namespace mp {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-template <class S, int W, int NC, bool ALLOW_STAGED>
+// WAVES: waves of the workgroup that share the staged weight stream (four: one
+// per SIMD; eight: two per SIMD, for single-column tiles whose registers and
+// LDS hand-off regions fit twice).
+template <class S, int W, int NC, bool ALLOW_STAGED, int WAVES = 4>
 struct LevelCore {
   typedef typename S::Frag Frag;
   typedef typename S::Acc Acc;
@@ -57,8 +60,9 @@ struct LevelCore {
   static_assert(STEPS % PF == 0, "the look-ahead ring must tile a layer");
   static_assert((size_t)(STAGED ? 2 * SF : PF) * 64 * sizeof(Frag) <= APK_PAD,
                 "the stream reads past the padding");
-  // LDS of a workgroup of four waves: ring, then one handoff region per wave
-  static constexpr size_t LDS_BYTES = (size_t)(RING_FRAGS + 4 * HANDOFF_FRAGS) * sizeof(Frag);
+  static_assert(SF % WAVES == 0, "every wave copies the same share of a stage");
+  // LDS of a workgroup: ring, then one handoff region per wave
+  static constexpr size_t LDS_BYTES = (size_t)(RING_FRAGS + WAVES * HANDOFF_FRAGS) * sizeof(Frag);
 
   // epilogue slices: accumulator registers retired per K step
   static constexpr int QPK = KBS >= CREGS ? 1 : CREGS / KBS;
@@ -74,12 +78,13 @@ struct LevelCore {
     int lane, wave;
     Frag pf[PF];
 
-    // Stage `stage` into ring slot `slot`: each of the four waves copies 4 of
-    // the stage's 16 fragments (one 16-byte direct load per lane each).
+    // Stage `stage` into ring slot `slot`: each of the WAVES waves copies
+    // SF / WAVES of the stage's 16 fragments (one 16-byte direct load per lane
+    // each).
     __device__ __forceinline__ void copy_stage(int64_t stage, int slot) {
 #pragma unroll
-      for (int i = 0; i < SF / 4; ++i) {
-        const int f = (SF / 4) * wave + i;
+      for (int i = 0; i < SF / WAVES; ++i) {
+        const int f = (SF / WAVES) * wave + i;
         __builtin_amdgcn_global_load_lds((const void*)(A + (stage * SF + f) * 64 + lane),
                                          (lds_void*)(ring + (slot * SF + f) * 64), 16, 0, 0);
       }
@@ -311,17 +316,17 @@ constexpr int default_cols(int W) {
   return S::ROWS == 32 ? (W == 64 ? 4 : W == 512 ? 1 : 2) : (W == 64 ? 8 : W == 128 ? 4 : W == 256 ? 2 : 1);
 }
 
-// Block `block` (4 waves x TILE points) of one level launch: evaluate, store
-// (accumulate), append in-band keys.  n: points of the launch (device count
-// for index lists), with block * 4 * TILE < n.  Every wave of the workgroup
-// calls it (the staged stream has barriers); a wave whose tile lies past n runs
-// on clamped points and stores nothing.
-template <class S, int W, int NC>
+// Block `block` (WAVES waves x TILE points) of one level launch: evaluate,
+// store (accumulate), append in-band keys.  n: points of the launch (device
+// count for index lists), with block * WAVES * TILE < n.  Every wave of the
+// workgroup calls it (the staged stream has barriers); a wave whose tile lies
+// past n runs on clamped points and stores nothing.
+template <class S, int W, int NC, int WAVES = 4>
 __device__ __forceinline__ void eval_block(const LevelDev& L, const EvalTarget& t, int64_t n, int64_t block,
                                   typename S::Frag* lds) {
-  typedef LevelCore<S, W, NC, true> M;
+  typedef LevelCore<S, W, NC, true, WAVES> M;
   const int lane = threadIdx.x & 63;
-  const int64_t base = (block * 4 + (threadIdx.x >> 6)) * M::TILE;
+  const int64_t base = (block * WAVES + (threadIdx.x >> 6)) * M::TILE;
 
   float x[NC][3];
   int64_t key[NC];

@@ -116,3 +116,31 @@ def test_gradient_of_level_sum():
     ref = sum(osiren.forward_grad(osiren.layers_from_model(m), x64.numpy())[1] for m in levels)
     _, g = LevelStack(levels, "fp32").eval(x64.float().cuda(), grad=True)
     assert np.abs(g.double().cpu().numpy() - ref).max() <= 2e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_single_column_tiles_equal_default_tiles(case, precision):
+    # single-column tiles run two waves per SIMD (eight-wave workgroups) up to
+    # width 256; each point's arithmetic is the same, so the values are too
+    hidden, omega, last = case
+    m = seeded_siren(hidden, omega, seed=4321 + hidden[0], last_scale=last).cuda()
+    x = _points(10007, 9).float().cuda()
+    st = LevelStack([m], precision)
+    ref = st.eval(x).clone()
+    st.set_tiles(1)
+    assert torch.equal(st.eval(x), ref)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_single_column_tiles_in_the_culled_extraction(precision):
+    levels = [seeded_siren([256] * 3, 30.0, seed=5, last_scale=20.0).cuda(),
+              seeded_siren([128] * 3, 60.0, seed=6, last_scale=0.1).cuda()]
+    st = LevelStack(levels, precision)
+    vol_a = torch.empty(96 ** 3, device="cuda")
+    vol_b = torch.empty(96 ** 3, device="cuda")
+    _, counts_a = st.extract_volume(96, [0.05], chunk=4096, out=vol_a)
+    st.set_tiles(1)
+    _, counts_b = st.extract_volume(96, [0.05], chunk=4096, out=vol_b)
+    assert counts_a == counts_b and counts_a[1] > 0
+    assert torch.equal(vol_a, vol_b)
