@@ -14,7 +14,7 @@
 // exclusive prefix sum of the per-point vertex counts in linear point order;
 // face ids the same over per-cell triangle counts.  Passes:
 //   1. count: per point (vertex count, triangle count), block sums
-//   2. scan the block sums (one workgroup)
+//   2. scan the block sums (segment sums, their scan, the segments)
 //   3. vertices: block-local scan + block offset -> vbase[point]; write vertices
 //   4. faces: recompute each cell's tiling, block-local scan -> face ids; map
 //      each tiling entry to (owner point, axis) -> vertex id
@@ -222,11 +222,29 @@ __device__ inline int mc_edge_flags(const McVol& V, int x, int y, int z) {
   return f;
 }
 
-__device__ inline McPoint mc_point(const McVol& V, int64_t i) {
+// (x, y, z) of point i = blockIdx.x * MC_BLOCK + threadIdx.x (fewer than 2^31
+// points): the block's first point is split once (uniform values), the thread
+// offset carried into it; at most MC_BLOCK / nx + 1 carries.
+__device__ inline void mc_xyz(const McVol& V, int& x, int& y, int& z) {
+  const uint32_t nx = (uint32_t)V.nx, ny = (uint32_t)V.ny;
+  const uint32_t b0 = blockIdx.x * (uint32_t)MC_BLOCK;
+  const uint32_t row = b0 / nx;
+  uint32_t xx = b0 - row * nx + threadIdx.x;
+  uint32_t yy = row % ny, zz = row / ny;
+  while (xx >= nx) {
+    xx -= nx;
+    if (++yy == ny) {
+      yy = 0;
+      ++zz;
+    }
+  }
+  x = (int)xx;
+  y = (int)yy;
+  z = (int)zz;
+}
+
+__device__ inline McPoint mc_point(const McVol& V, int x, int y, int z) {
   McPoint r{0, 0, 0};
-  const int x = (int)(i % V.nx);
-  const int y = (int)((i / V.nx) % V.ny);
-  const int z = (int)(i / ((int64_t)V.nx * V.ny));
   r.flags = mc_edge_flags(V, x, y, z);
   r.nv = __popc(r.flags);
   if (x + 1 < V.nx && y + 1 < V.ny && z + 1 < V.nz) {
@@ -270,7 +288,11 @@ __device__ inline void block_scan2(int a, int b, int* ea, int* eb, int* ta, int*
 __global__ __launch_bounds__(MC_BLOCK) void k_mc_count(McVol V, int64_t n, uint32_t* block_tot) {
   const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
   McPoint p{0, 0, 0};
-  if (i < n) p = mc_point(V, i);
+  if (i < n) {
+    int x, y, z;
+    mc_xyz(V, x, y, z);
+    p = mc_point(V, x, y, z);
+  }
   int ea, eb, ta, tb;
   block_scan2(p.nv, p.nt, &ea, &eb, &ta, &tb);
   if (threadIdx.x == 0) {
@@ -279,16 +301,52 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_count(McVol V, int64_t n, uint3
   }
 }
 
-// Exclusive scan of the block sums in place (one workgroup of 1024 threads),
-// totals into tot[0..1] as 64-bit values.
-__global__ __launch_bounds__(1024) void k_mc_scan(uint32_t* block_tot, int64_t nblocks,
-                                                  unsigned long long* tot) {
+// Exclusive scan of the block sums (pairs: vertices, triangles) in place, in
+// three passes: segment sums (MC_SEG blocks per workgroup), one workgroup
+// scanning the segment sums (totals into tot[0..1]), then every segment
+// scanned from its offset.
+static constexpr int MC_SEG = 8 * MC_BLOCK;
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_scan_reduce(const uint32_t* __restrict__ block_tot, int64_t nblocks,
+                                                             unsigned long long* __restrict__ seg) {
+  const int64_t s0 = (int64_t)blockIdx.x * MC_SEG;
+  unsigned long long a = 0, b = 0;
+  for (int k = threadIdx.x; k < MC_SEG; k += MC_BLOCK) {
+    const int64_t j = s0 + k;
+    if (j < nblocks) {
+      a += block_tot[2 * j];
+      b += block_tot[2 * j + 1];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  __shared__ unsigned long long sa[MC_BLOCK / 64], sb[MC_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long ta = 0, tb = 0;
+    for (int w = 0; w < MC_BLOCK / 64; ++w) {
+      ta += sa[w];
+      tb += sb[w];
+    }
+    seg[2 * blockIdx.x] = ta;
+    seg[2 * blockIdx.x + 1] = tb;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_mc_scan_top(unsigned long long* seg, int64_t nseg,
+                                                      unsigned long long* tot) {
   __shared__ unsigned long long s0[1024], s1[1024];
   const int t = threadIdx.x;
-  const int64_t per = (nblocks + 1023) / 1024;
-  const int64_t lo = t * per, hi = (lo + per < nblocks) ? lo + per : nblocks;
+  const int64_t per = (nseg + 1023) / 1024;
+  const int64_t lo = t * per, hi = (lo + per < nseg) ? lo + per : nseg;
   unsigned long long a = 0, b = 0;
-  for (int64_t k = lo; k < hi; ++k) { a += block_tot[2 * k]; b += block_tot[2 * k + 1]; }
+  for (int64_t k = lo; k < hi; ++k) { a += seg[2 * k]; b += seg[2 * k + 1]; }
   s0[t] = a;
   s1[t] = b;
   __syncthreads();
@@ -302,13 +360,44 @@ __global__ __launch_bounds__(1024) void k_mc_scan(uint32_t* block_tot, int64_t n
   }
   unsigned long long oa = s0[t] - a, ob = s1[t] - b;
   for (int64_t k = lo; k < hi; ++k) {
-    const uint32_t ca = block_tot[2 * k], cb = block_tot[2 * k + 1];
-    block_tot[2 * k] = (uint32_t)oa;
-    block_tot[2 * k + 1] = (uint32_t)ob;
+    const unsigned long long ca = seg[2 * k], cb = seg[2 * k + 1];
+    seg[2 * k] = oa;
+    seg[2 * k + 1] = ob;
     oa += ca;
     ob += cb;
   }
   if (t == 1023) { tot[0] = s0[1023]; tot[1] = s1[1023]; }
+}
+
+// Segment blockIdx.x: each thread scans 8 consecutive block sums serially,
+// the workgroup scans the thread totals, then every entry gets its offset.
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_scan_apply(uint32_t* __restrict__ block_tot, int64_t nblocks,
+                                                            const unsigned long long* __restrict__ seg) {
+  constexpr int PER = MC_SEG / MC_BLOCK;
+  const int64_t j0 = (int64_t)blockIdx.x * MC_SEG + (int64_t)threadIdx.x * PER;
+  uint32_t ca[PER], cb[PER];
+  uint32_t a = 0, b = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int64_t j = j0 + k;
+    ca[k] = j < nblocks ? block_tot[2 * j] : 0u;
+    cb[k] = j < nblocks ? block_tot[2 * j + 1] : 0u;
+    a += ca[k];
+    b += cb[k];
+  }
+  int ea, eb, ta, tb;
+  block_scan2((int)a, (int)b, &ea, &eb, &ta, &tb);
+  unsigned long long oa = seg[2 * blockIdx.x] + (unsigned)ea, ob = seg[2 * blockIdx.x + 1] + (unsigned)eb;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int64_t j = j0 + k;
+    if (j < nblocks) {
+      block_tot[2 * j] = (uint32_t)oa;
+      block_tot[2 * j + 1] = (uint32_t)ob;
+    }
+    oa += ca[k];
+    ob += cb[k];
+  }
 }
 
 __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, int64_t n, const uint32_t* block_off,
@@ -316,16 +405,17 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, int64_t n, co
                                                           float* values) {
   const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
   McPoint p{0, 0, 0};
-  if (i < n) p = mc_point(V, i);
+  int x = 0, y = 0, z = 0;
+  if (i < n) {
+    mc_xyz(V, x, y, z);
+    p = mc_point(V, x, y, z);
+  }
   int ea, eb, ta, tb;
   block_scan2(p.nv, 0, &ea, &eb, &ta, &tb);
   if (i >= n) return;
   const uint32_t base = block_off[2 * blockIdx.x] + (uint32_t)ea;
   vbase[i] = base;
   if (p.nv == 0) return;
-  const int x = (int)(i % V.nx);
-  const int y = (int)((i / V.nx) % V.ny);
-  const int z = (int)(i / ((int64_t)V.nx * V.ny));
   const double v0 = mc_value(V, x, y, z);
   uint32_t id = base;
   for (int ax = 0; ax < 3; ++ax) {
@@ -414,9 +504,7 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_faces(McVol V, int64_t n, const
   int x = 0, y = 0, z = 0;
   const int8_t* tri = nullptr;
   if (i < n) {
-    x = (int)(i % V.nx);
-    y = (int)((i / V.nx) % V.ny);
-    z = (int)(i / ((int64_t)V.nx * V.ny));
+    mc_xyz(V, x, y, z);
     if (x + 1 < V.nx && y + 1 < V.ny && z + 1 < V.nz) {
       idx = mc_load_cell(V, x, y, z, v, raw);
       if (idx != 0 && idx != 255) nt = mc_tiling(v, idx, &tri);
@@ -435,9 +523,12 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_faces(McVol V, int64_t n, const
 
 static size_t mc_blocks(int64_t n) { return (size_t)((n + MC_BLOCK - 1) / MC_BLOCK); }
 
-static size_t mc_ws_bytes(int64_t n) {
-  return 256 + ((mc_blocks(n) * 8 + 255) & ~(size_t)255) + (size_t)n * 4;
-}
+static size_t mc_segs(int64_t n) { return (mc_blocks(n) + MC_SEG - 1) / MC_SEG; }
+
+// workspace: totals (256 B) | block sums [blocks][2] | vertex bases [n] | segment sums [segs][2]
+static size_t mc_vbase_off(int64_t n) { return 256 + ((mc_blocks(n) * 8 + 255) & ~(size_t)255); }
+static size_t mc_seg_off(int64_t n) { return (mc_vbase_off(n) + (size_t)n * 4 + 255) & ~(size_t)255; }
+static size_t mc_ws_bytes(int64_t n) { return mc_seg_off(n) + mc_segs(n) * 16; }
 
 static int mc_check(const float* vol, int nz, int ny, int nx, void* ws, const char* fn) {
   MP_REQUIRE(vol != nullptr && ws != nullptr, MP_EINVAL, "%s: vol and ws must be non-NULL", fn);
@@ -471,8 +562,14 @@ int mp_mc_count(const float* vol, int nz, int ny, int nx, float level, int64_t* 
   const size_t nb = mc_blocks(n);
   hipLaunchKernelGGL(k_mc_count, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block);
   MP_LAUNCHED("k_mc_count");
-  hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, block, (int64_t)nb, tot);
-  MP_LAUNCHED("k_mc_scan");
+  unsigned long long* seg = reinterpret_cast<unsigned long long*>(w + mc_seg_off(n));
+  const size_t ns = mc_segs(n);
+  hipLaunchKernelGGL(k_mc_scan_reduce, dim3((unsigned)ns), dim3(MC_BLOCK), 0, s, block, (int64_t)nb, seg);
+  MP_LAUNCHED("k_mc_scan_reduce");
+  hipLaunchKernelGGL(k_mc_scan_top, dim3(1), dim3(1024), 0, s, seg, (int64_t)ns, tot);
+  MP_LAUNCHED("k_mc_scan_top");
+  hipLaunchKernelGGL(k_mc_scan_apply, dim3((unsigned)ns), dim3(MC_BLOCK), 0, s, block, (int64_t)nb, seg);
+  MP_LAUNCHED("k_mc_scan_apply");
   unsigned long long h[2] = {0, 0};
   MP_HIP(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, s));
   MP_HIP(hipStreamSynchronize(s));
@@ -494,7 +591,7 @@ int mp_mc_emit(const float* vol, int nz, int ny, int nx, float level, float* ver
   char* w = reinterpret_cast<char*>(ws);
   const size_t nb = mc_blocks(n);
   const uint32_t* block = reinterpret_cast<const uint32_t*>(w + 256);
-  uint32_t* vbase = reinterpret_cast<uint32_t*>(w + 256 + ((nb * 8 + 255) & ~(size_t)255));
+  uint32_t* vbase = reinterpret_cast<uint32_t*>(w + mc_vbase_off(n));
   hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, vbase,
                      verts, normals, values);
   MP_LAUNCHED("k_mc_vertices");
