@@ -44,3 +44,29 @@ def test_abi_version_and_error_path(lib):
         assert "N must be" in str(e)
     else:
         raise AssertionError("check() did not raise")
+
+
+def test_training_data_and_renderer_arguments_are_checked_on_the_host(lib):
+    import numpy as np
+    from mplicits import _lib
+    _lib.load()  # argument tables
+    out = ctypes.c_void_p()
+    v = np.zeros((3, 3), np.float32)
+    f = np.array([[0, 1, 3]], np.int32)  # vertex 3 of 3
+    rc = lib.mp_mesh_create(v.ctypes.data_as(ctypes.c_void_p), 3, f.ctypes.data_as(ctypes.c_void_p), 1, None,
+                            ctypes.byref(out))
+    assert rc == _lib.MP_EINVAL and b"indexes vertex 3" in lib.mp_last_error() and not out.value
+    rc = lib.mp_mesh_create(v.ctypes.data_as(ctypes.c_void_p), 3, f.ctypes.data_as(ctypes.c_void_p), 0, None,
+                            ctypes.byref(out))
+    assert rc == _lib.MP_EINVAL and b"empty mesh" in lib.mp_last_error()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the checks come first
+    rc = lib.mp_neighborhood_samples(dummy, 10, 6, None, dummy, 4, 0.02, 5e-4, 1, 1, dummy, dummy, dummy, None)
+    assert rc == _lib.MP_EINVAL and b"seventh column" in lib.mp_last_error()
+    rc = lib.mp_neighborhood_samples(dummy, 10, 6, None, dummy, 11, 0.02, 5e-4, 0, 1, dummy, dummy, dummy, None)
+    assert rc == _lib.MP_EINVAL and b"without ids" in lib.mp_last_error()
+    rc = lib.mp_train_dw(dummy, dummy, 96, 256, 256, 1, dummy, None)
+    assert rc == _lib.MP_EINVAL and b"multiple of 64" in lib.mp_last_error()
+    rc = lib.mp_train_dw(dummy, dummy, 128, 256, 256, 0, dummy, None)
+    assert rc == _lib.MP_EINVAL and b"bfloat16 (1) or float16 (2)" in lib.mp_last_error()
+    rc = lib.mp_render(None, dummy, 1, None, dummy, None, None, None, None, None)
+    assert rc == _lib.MP_EINVAL and b"NULL argument" in lib.mp_last_error()
