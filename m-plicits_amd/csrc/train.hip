@@ -450,6 +450,54 @@ __global__ __launch_bounds__(256) void k_train_out_fwd(float* __restrict__ Z, co
   }
 }
 
+// The same with 16 lanes per point (W % 64 == 0): each lane reads 4 units
+// per row with one 16-byte load, a row's 16 lanes 256 contiguous bytes, and
+// the sums reduce over 16 lanes; 16 points per block.
+template <int NT>
+__global__ __launch_bounds__(256) void k_train_out_fwd16(float* __restrict__ Z, const float* __restrict__ b_sine,
+                                                         const float* __restrict__ wl, const float* __restrict__ bl,
+                                                         int64_t n, int W, float* __restrict__ f,
+                                                         float* __restrict__ fd) {
+  const int64_t p = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int j = threadIdx.x & 15;
+  if (p >= n) return;  // the whole 16-lane group
+  float acc[1 + NT];
+#pragma unroll
+  for (int k = 0; k <= NT; ++k) acc[k] = 0.f;
+  float* z = Z + p * (1 + NT) * W;
+  for (int u = 4 * j; u < W; u += 64) {
+    float4 a = ld4(z, u);
+    if (b_sine) {
+      const float4 bb = ld4(b_sine, u);
+      a = make_float4(a.x + bb.x, a.y + bb.y, a.z + bb.z, a.w + bb.w);
+      *reinterpret_cast<float4*>(z + u) = a;
+    }
+    const float4 w4 = ld4(wl, u);
+    const float av[4] = {a.x, a.y, a.z, a.w}, wv[4] = {w4.x, w4.y, w4.z, w4.w};
+    float sn[4], cs[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sincos_reduced(av[e], sn[e], cs[e]);
+      acc[0] = fmaf(wv[e], sn[e], acc[0]);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const float4 zt = ld4(z, (int64_t)(1 + t) * W + u);
+      const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[1 + t] = fmaf(wv[e] * cs[e], zv[e], acc[1 + t]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k <= NT; ++k)
+    for (int o = 8; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+  if (j == 0) {
+    f[p] = acc[0] + bl[0];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) fd[p * NT + t] = acc[1 + t];
+  }
+}
+
 // grid.x = ceil(n / rows_per_block), block = 256 threads over W (W <= 512:
 // two units per thread); dwl_part [gridDim.x][W].
 __global__ __launch_bounds__(256) void k_train_out_bwd(const float* __restrict__ Z, const float* __restrict__ s,
@@ -716,7 +764,19 @@ int mp_train_out_fwd(float* Z, const float* b_sine, const float* w_last, const f
              "mp_train_out_fwd: bad arguments");
   MP_REQUIRE(n_tangents == 0 || fd, MP_EINVAL, "mp_train_out_fwd: fd is NULL");
   if (n == 0) return MP_OK;
-  hipLaunchKernelGGL(k_train_out_fwd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, as_stream(stream), Z, b_sine,
+  hipStream_t st = as_stream(stream);
+  if (W % 64 == 0) {
+    const dim3 g((unsigned)((n + 15) / 16));
+    switch (n_tangents) {
+      case 0: hipLaunchKernelGGL(k_train_out_fwd16<0>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
+      case 1: hipLaunchKernelGGL(k_train_out_fwd16<1>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
+      case 2: hipLaunchKernelGGL(k_train_out_fwd16<2>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
+      default: hipLaunchKernelGGL(k_train_out_fwd16<3>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
+    }
+    MP_LAUNCHED("k_train_out_fwd16");
+    return MP_OK;
+  }
+  hipLaunchKernelGGL(k_train_out_fwd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, Z, b_sine,
                      w_last, b_last, n, W, n_tangents, f, fd);
   MP_LAUNCHED("k_train_out_fwd");
   return MP_OK;
