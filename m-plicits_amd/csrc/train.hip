@@ -145,85 +145,120 @@ __device__ inline void st_blocked(T* Xb, int64_t p, int W, int u, const float (&
   }
 }
 
-template <class T, int NT>
+// Two consecutive points (8 rows, point pair `pair`) at once: one 16-byte
+// store per unit.  lo / hi: rows of the even / odd point.
+template <class T>
+__device__ inline void st_blocked8(T* Xb, int64_t pair, int W, int u, const float (&lo)[4][4],
+                                   const float (&hi)[4][4]) {
+  typedef T v8 __attribute__((ext_vector_type(8)));
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v8 o;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      o[t] = (T)lo[t][e];
+      o[4 + t] = (T)hi[t][e];
+    }
+    *reinterpret_cast<v8*>(Xb + (pair * W + u + e) * 8) = o;
+  }
+}
+
+// Blocked copy of P points' rows (P = 1: the point's half of each 16-byte
+// chunk; P = 2: whole chunks).
+template <class T, int P>
+__device__ inline void st_blocked_p(T* Xb, int64_t p_first, int W, int u, const float (&rows)[P][4][4]) {
+  if constexpr (P == 2) st_blocked8(Xb, p_first >> 1, W, u, rows[0], rows[1]);
+  else st_blocked(Xb, p_first, W, u, rows[0]);
+}
+
+template <class T, int NT, int P>
 __global__ __launch_bounds__(256) void k_train_first4(const float* __restrict__ x, const float* __restrict__ v,
                                                       const float* __restrict__ W0, const float* __restrict__ b0,
                                                       int64_t n, int W, float* __restrict__ Z, T* __restrict__ X,
                                                       T* __restrict__ Xb) {
+  // P consecutive points x 4 units per thread
   const int W4 = W / 4;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * W4) return;
-  const int64_t p = q / W4;
-  const int u = (int)(q - p * W4) * 4;
-  const int64_t base = p * (1 + NT) * W + u;
-  const float x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
-  float a[4], s[4], c[4];
+  if (q >= (n / P) * W4) return;
+  const int64_t pq = q / W4;
+  const int u = (int)(q - pq * W4) * 4;
+  float rows[P][4][4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const float* w = W0 + 3 * (u + e);
-    a[e] = fmaf(w[2], x2, fmaf(w[1], x1, fmaf(w[0], x0, b0[u + e])));
-    sincos_reduced(a[e], s[e], c[e]);
-  }
-  *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
-  st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
-  float rows[4][4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) rows[0][e] = s[e];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const float* vt = v + (p * NT + t) * 3;
-    float ad[4];
+  for (int pp = 0; pp < P; ++pp) {
+    const int64_t p = pq * P + pp;
+    const int64_t base = p * (1 + NT) * W + u;
+    const float x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
+    float a[4], s[4], c[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float* w = W0 + 3 * (u + e);
-      ad[e] = fmaf(w[2], vt[2], fmaf(w[1], vt[1], w[0] * vt[0]));
+      a[e] = fmaf(w[2], x2, fmaf(w[1], x1, fmaf(w[0], x0, b0[u + e])));
+      sincos_reduced(a[e], s[e], c[e]);
     }
-    const int64_t k = base + (int64_t)(1 + t) * W;
-    *reinterpret_cast<float4*>(Z + k) = make_float4(ad[0], ad[1], ad[2], ad[3]);
-    st4(X, k, make_float4(c[0] * ad[0], c[1] * ad[1], c[2] * ad[2], c[3] * ad[3]));
-    if constexpr (NT == 3) {
+    *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
+    st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
 #pragma unroll
-      for (int e = 0; e < 4; ++e) rows[1 + t][e] = c[e] * ad[e];
+    for (int e = 0; e < 4; ++e) rows[pp][0][e] = s[e];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const float* vt = v + (p * NT + t) * 3;
+      float ad[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float* w = W0 + 3 * (u + e);
+        ad[e] = fmaf(w[2], vt[2], fmaf(w[1], vt[1], w[0] * vt[0]));
+      }
+      const int64_t k = base + (int64_t)(1 + t) * W;
+      *reinterpret_cast<float4*>(Z + k) = make_float4(ad[0], ad[1], ad[2], ad[3]);
+      st4(X, k, make_float4(c[0] * ad[0], c[1] * ad[1], c[2] * ad[2], c[3] * ad[3]));
+      if constexpr (NT == 3) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = c[e] * ad[e];
+      }
     }
   }
   if constexpr (NT == 3) {
-    if (Xb) st_blocked(Xb, p, W, u, rows);
+    if (Xb) st_blocked_p<T, P>(Xb, pq * P, W, u, rows);
   }
 }
 
-template <class T, int NT>
+template <class T, int NT, int P>
 __global__ __launch_bounds__(256) void k_train_act4(float* __restrict__ Z, const float* __restrict__ b, int64_t n,
                                                     int W, T* __restrict__ X, T* __restrict__ Xb) {
+  // P consecutive points x 4 units per thread
   const int W4 = W / 4;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * W4) return;
-  const int64_t p = q / W4;
-  const int u = (int)(q - p * W4) * 4;
-  const int64_t base = p * (1 + NT) * W + u;
-  const float4 z = ld4(Z, base);
+  if (q >= (n / P) * W4) return;
+  const int64_t pq = q / W4;
+  const int u = (int)(q - pq * W4) * 4;
   const float4 bb = *reinterpret_cast<const float4*>(b + u);
-  const float a[4] = {z.x + bb.x, z.y + bb.y, z.z + bb.z, z.w + bb.w};
-  float s[4], c[4];
+  float rows[P][4][4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) sincos_reduced(a[e], s[e], c[e]);
-  *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
-  st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
-  float rows[4][4];
+  for (int pp = 0; pp < P; ++pp) {
+    const int64_t base = (pq * P + pp) * (1 + NT) * W + u;
+    const float4 z = ld4(Z, base);
+    const float a[4] = {z.x + bb.x, z.y + bb.y, z.z + bb.z, z.w + bb.w};
+    float s[4], c[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) rows[0][e] = s[e];
+    for (int e = 0; e < 4; ++e) sincos_reduced(a[e], s[e], c[e]);
+    *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
+    st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int64_t k = base + (int64_t)(1 + t) * W;
-    const float4 ad = ld4(Z, k);
-    const float o[4] = {c[0] * ad.x, c[1] * ad.y, c[2] * ad.z, c[3] * ad.w};
-    st4(X, k, make_float4(o[0], o[1], o[2], o[3]));
-    if constexpr (NT == 3) {
+    for (int e = 0; e < 4; ++e) rows[pp][0][e] = s[e];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+    for (int t = 0; t < NT; ++t) {
+      const int64_t k = base + (int64_t)(1 + t) * W;
+      const float4 ad = ld4(Z, k);
+      const float o[4] = {c[0] * ad.x, c[1] * ad.y, c[2] * ad.z, c[3] * ad.w};
+      st4(X, k, make_float4(o[0], o[1], o[2], o[3]));
+      if constexpr (NT == 3) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = o[e];
+      }
     }
   }
   if constexpr (NT == 3) {
-    if (Xb) st_blocked(Xb, p, W, u, rows);
+    if (Xb) st_blocked_p<T, P>(Xb, pq * P, W, u, rows);
   }
 }
 
@@ -284,6 +319,17 @@ static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256);
     }                                                                                         \
   } while (0)
 
+// The same for kernels with P points per thread.
+#define MP_TRAIN_NTP_DISPATCH(KERNEL, T, P, nt, grid, s, ...)                                       \
+  do {                                                                                              \
+    switch (nt) {                                                                                   \
+      case 0: hipLaunchKernelGGL((KERNEL<T, 0, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;   \
+      case 1: hipLaunchKernelGGL((KERNEL<T, 1, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;   \
+      case 2: hipLaunchKernelGGL((KERNEL<T, 2, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;   \
+      default: hipLaunchKernelGGL((KERNEL<T, 3, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;  \
+    }                                                                                               \
+  } while (0)
+
 }  // namespace mp
 
 using namespace mp;
@@ -300,16 +346,23 @@ int mp_train_first(const float* x, const float* v, const float* W0, const float*
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
   if (W % 4 == 0) {
-    const unsigned g = grid_for(n * (W / 4));
-    if (x_dtype == 1)
-      MP_TRAIN_NT_DISPATCH(k_train_first4, __bf16, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<__bf16*>(X),
-                           reinterpret_cast<__bf16*>(Xb));
+    // with the blocked copy: two points per thread (whole 16-byte chunks)
+    const unsigned g = grid_for((Xb ? n / 2 : n) * (W / 4));
+    if (x_dtype == 1 && Xb)
+      MP_TRAIN_NTP_DISPATCH(k_train_first4, __bf16, 2, n_tangents, g, s, x, v, W0, b0, n, W, Z,
+                            reinterpret_cast<__bf16*>(X), reinterpret_cast<__bf16*>(Xb));
+    else if (x_dtype == 1)
+      MP_TRAIN_NTP_DISPATCH(k_train_first4, __bf16, 1, n_tangents, g, s, x, v, W0, b0, n, W, Z,
+                            reinterpret_cast<__bf16*>(X), static_cast<__bf16*>(nullptr));
+    else if (x_dtype == 2 && Xb)
+      MP_TRAIN_NTP_DISPATCH(k_train_first4, _Float16, 2, n_tangents, g, s, x, v, W0, b0, n, W, Z,
+                            reinterpret_cast<_Float16*>(X), reinterpret_cast<_Float16*>(Xb));
     else if (x_dtype == 2)
-      MP_TRAIN_NT_DISPATCH(k_train_first4, _Float16, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                           reinterpret_cast<_Float16*>(X), reinterpret_cast<_Float16*>(Xb));
+      MP_TRAIN_NTP_DISPATCH(k_train_first4, _Float16, 1, n_tangents, g, s, x, v, W0, b0, n, W, Z,
+                            reinterpret_cast<_Float16*>(X), static_cast<_Float16*>(nullptr));
     else
-      MP_TRAIN_NT_DISPATCH(k_train_first4, float, n_tangents, g, s, x, v, W0, b0, n, W, Z, reinterpret_cast<float*>(X),
-                           static_cast<float*>(nullptr));
+      MP_TRAIN_NTP_DISPATCH(k_train_first4, float, 1, n_tangents, g, s, x, v, W0, b0, n, W, Z,
+                            reinterpret_cast<float*>(X), static_cast<float*>(nullptr));
     MP_LAUNCHED("k_train_first4");
     return MP_OK;
   }
@@ -335,16 +388,23 @@ int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, voi
   if (n == 0) return MP_OK;
   hipStream_t s = as_stream(stream);
   if (W % 4 == 0) {
-    const unsigned g = grid_for(n * (W / 4));
-    if (x_dtype == 1)
-      MP_TRAIN_NT_DISPATCH(k_train_act4, __bf16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X),
-                           reinterpret_cast<__bf16*>(Xb));
+    // with the blocked copy: two points per thread (whole 16-byte chunks)
+    const unsigned g = grid_for((Xb ? n / 2 : n) * (W / 4));
+    if (x_dtype == 1 && Xb)
+      MP_TRAIN_NTP_DISPATCH(k_train_act4, __bf16, 2, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X),
+                            reinterpret_cast<__bf16*>(Xb));
+    else if (x_dtype == 1)
+      MP_TRAIN_NTP_DISPATCH(k_train_act4, __bf16, 1, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X),
+                            static_cast<__bf16*>(nullptr));
+    else if (x_dtype == 2 && Xb)
+      MP_TRAIN_NTP_DISPATCH(k_train_act4, _Float16, 2, n_tangents, g, s, Z, b, n, W,
+                            reinterpret_cast<_Float16*>(X), reinterpret_cast<_Float16*>(Xb));
     else if (x_dtype == 2)
-      MP_TRAIN_NT_DISPATCH(k_train_act4, _Float16, n_tangents, g, s, Z, b, n, W, reinterpret_cast<_Float16*>(X),
-                           reinterpret_cast<_Float16*>(Xb));
+      MP_TRAIN_NTP_DISPATCH(k_train_act4, _Float16, 1, n_tangents, g, s, Z, b, n, W,
+                            reinterpret_cast<_Float16*>(X), static_cast<_Float16*>(nullptr));
     else
-      MP_TRAIN_NT_DISPATCH(k_train_act4, float, n_tangents, g, s, Z, b, n, W, reinterpret_cast<float*>(X),
-                           static_cast<float*>(nullptr));
+      MP_TRAIN_NTP_DISPATCH(k_train_act4, float, 1, n_tangents, g, s, Z, b, n, W, reinterpret_cast<float*>(X),
+                            static_cast<float*>(nullptr));
     MP_LAUNCHED("k_train_act4");
     return MP_OK;
   }
@@ -576,7 +636,7 @@ __global__ __launch_bounds__(128) void k_train_out_bwd4(const float* __restrict_
 // are written as the GEMM operand (and its K-blocked copy), and
 // part [blocks][2][W] receives per-block partial sums of
 // dw_L = sum s sin a + sum_t g_t cos a ad_t (row 0) and db = sum abar (row 1).
-template <class T, int NT>
+template <class T, int NT, int P>
 __global__ __launch_bounds__(128) void k_train_out_act_bwd4(const float* __restrict__ Z, const float* __restrict__ s,
                                                             const float* __restrict__ g, const float* __restrict__ wl,
                                                             int64_t n, int W, T* __restrict__ Abar, T* __restrict__ Ab,
@@ -587,50 +647,51 @@ __global__ __launch_bounds__(128) void k_train_out_act_bwd4(const float* __restr
   const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
   float dwl[4] = {0.f, 0.f, 0.f, 0.f}, db[4] = {0.f, 0.f, 0.f, 0.f};
   const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
-  for (int r = 0; r < RED_ROWS; ++r) {
-    const int64_t p = p0 + r;
-    if (p >= n) break;
-    const int64_t base = p * (1 + NT) * W + u;
-    const float4 z = ld4(Z, base);
-    const float za[4] = {z.x, z.y, z.z, z.w};
-    float sn[4], cs[4], abar[4];
-    const float sv = s ? s[p] : 0.f;
+  for (int r = 0; r < RED_ROWS; r += P) {
+    if (p0 + r >= n) break;  // n is even when P = 2
+    float rows[P][4][4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      sincos_reduced(za[e], sn[e], cs[e]);
-      abar[e] = cs[e] * (sv * wv[e]);
-      dwl[e] = fmaf(sv, sn[e], dwl[e]);
-    }
-    float rows[4][4];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int64_t k = base + (int64_t)(1 + t) * W;
-      const float gv = g ? g[p * NT + t] : 0.f;
-      const float4 zt = ld4(Z, k);
-      const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
-      float o[4];
+    for (int pp = 0; pp < P; ++pp) {
+      const int64_t p = p0 + r + pp;
+      const int64_t base = p * (1 + NT) * W + u;
+      const float4 z = ld4(Z, base);
+      const float za[4] = {z.x, z.y, z.z, z.w};
+      float sn[4], cs[4], abar[4];
+      const float sv = s ? s[p] : 0.f;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float pt = gv * wv[e];
-        abar[e] = fmaf(-sn[e] * zv[e], pt, abar[e]);
-        o[e] = cs[e] * pt;
-        dwl[e] = fmaf(gv * cs[e], zv[e], dwl[e]);
+        sincos_reduced(za[e], sn[e], cs[e]);
+        abar[e] = cs[e] * (sv * wv[e]);
+        dwl[e] = fmaf(sv, sn[e], dwl[e]);
       }
-      st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
-      if constexpr (NT == 3) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+      for (int t = 0; t < NT; ++t) {
+        const int64_t k = base + (int64_t)(1 + t) * W;
+        const float gv = g ? g[p * NT + t] : 0.f;
+        const float4 zt = ld4(Z, k);
+        const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float pt = gv * wv[e];
+          abar[e] = fmaf(-sn[e] * zv[e], pt, abar[e]);
+          o[e] = cs[e] * pt;
+          dwl[e] = fmaf(gv * cs[e], zv[e], dwl[e]);
+        }
+        st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
+        if constexpr (NT == 3) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = o[e];
+        }
       }
+      st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) db[e] += abar[e];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rows[pp][0][e] = abar[e];
     }
-    st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
-#pragma unroll
-    for (int e = 0; e < 4; ++e) db[e] += abar[e];
     if constexpr (NT == 3) {
-      if (Ab) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) rows[0][e] = abar[e];
-        st_blocked(Ab, p, W, u, rows);
-      }
+      if (Ab) st_blocked_p<T, P>(Ab, p0 + r, W, u, rows);
     }
   }
   float* o = part + (int64_t)blockIdx.x * 2 * W + u;
@@ -641,54 +702,55 @@ __global__ __launch_bounds__(128) void k_train_out_act_bwd4(const float* __restr
 // Hidden-layer backward through sin / the tangent products (as
 // k_train_act_bwd4) with the bias gradient reduced per block: blockDim =
 // W / 4, db_part [blocks][W].  P: element type of Pbar.
-template <class T, class P, int NT>
+template <class T, class PB, int NT, int P>
 __global__ __launch_bounds__(128) void k_train_act_bwd_part4(const float* __restrict__ Z,
-                                                             const P* __restrict__ Pbar, int64_t n, int W,
+                                                             const PB* __restrict__ Pbar, int64_t n, int W,
                                                              T* __restrict__ Abar, T* __restrict__ Ab,
                                                              float* __restrict__ db_part) {
   const int u = threadIdx.x * 4;
   if (u >= W) return;
   float db[4] = {0.f, 0.f, 0.f, 0.f};
   const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
-  for (int r = 0; r < RED_ROWS; ++r) {
-    const int64_t p = p0 + r;
-    if (p >= n) break;
-    const int64_t base = p * (1 + NT) * W + u;
-    const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
-    const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
-    float sn[4], cs[4], abar[4];
+  for (int r = 0; r < RED_ROWS; r += P) {
+    if (p0 + r >= n) break;  // n is even when P = 2
+    float rows[P][4][4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      sincos_reduced(za[e], sn[e], cs[e]);
-      abar[e] = cs[e] * pa[e];
-    }
-    float rows[4][4];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int64_t k = base + (int64_t)(1 + t) * W;
-      const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
-      const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
-      float o[4];
+    for (int pp = 0; pp < P; ++pp) {
+      const int64_t p = p0 + r + pp;
+      const int64_t base = p * (1 + NT) * W + u;
+      const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
+      const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
+      float sn[4], cs[4], abar[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        abar[e] = fmaf(-sn[e] * zv[e], pv[e], abar[e]);
-        o[e] = cs[e] * pv[e];
+        sincos_reduced(za[e], sn[e], cs[e]);
+        abar[e] = cs[e] * pa[e];
       }
-      st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
-      if constexpr (NT == 3) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
+      for (int t = 0; t < NT; ++t) {
+        const int64_t k = base + (int64_t)(1 + t) * W;
+        const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
+        const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          abar[e] = fmaf(-sn[e] * zv[e], pv[e], abar[e]);
+          o[e] = cs[e] * pv[e];
+        }
+        st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
+        if constexpr (NT == 3) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = o[e];
+        }
       }
+      st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) db[e] += abar[e];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rows[pp][0][e] = abar[e];
     }
-    st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
-#pragma unroll
-    for (int e = 0; e < 4; ++e) db[e] += abar[e];
     if constexpr (NT == 3) {
-      if (Ab) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) rows[0][e] = abar[e];
-        st_blocked(Ab, p, W, u, rows);
-      }
+      if (Ab) st_blocked_p<T, P>(Ab, p0 + r, W, u, rows);
     }
   }
   *reinterpret_cast<float4*>(db_part + (int64_t)blockIdx.x * W + u) = make_float4(db[0], db[1], db[2], db[3]);
@@ -807,6 +869,17 @@ int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float
   return MP_OK;
 }
 
+// Launch K<T, NT, P> over `blocks` blocks of W / 4 threads.
+#define MP_TRAIN_REDP_DISPATCH(KERNEL, T, P, nt, blocks, W, s, ...)                                                  \
+  do {                                                                                                              \
+    switch (nt) {                                                                                                   \
+      case 0: hipLaunchKernelGGL((KERNEL<T, 0, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+      case 1: hipLaunchKernelGGL((KERNEL<T, 1, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL((KERNEL<T, 2, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<T, 3, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
+    }                                                                                                               \
+  } while (0)
+
 // Launch K<T, NT> over `blocks` blocks of W / 4 threads.
 #define MP_TRAIN_RED_DISPATCH(KERNEL, T, nt, blocks, W, s, ...)                                              \
   do {                                                                                                      \
@@ -830,15 +903,22 @@ int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const f
   if (n == 0) return MP_OK;
   const int64_t blocks = mp_train_out_bwd_blocks(n);
   hipStream_t st = as_stream(stream);
-  if (a_dtype == 1)
-    MP_TRAIN_RED_DISPATCH(k_train_out_act_bwd4, __bf16, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                          reinterpret_cast<__bf16*>(Abar), reinterpret_cast<__bf16*>(Ab), part);
+  // with the blocked copy: two points per step (whole 16-byte chunks)
+  if (a_dtype == 1 && Ab)
+    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, __bf16, 2, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                           reinterpret_cast<__bf16*>(Abar), reinterpret_cast<__bf16*>(Ab), part);
+  else if (a_dtype == 1)
+    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, __bf16, 1, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                           reinterpret_cast<__bf16*>(Abar), static_cast<__bf16*>(nullptr), part);
+  else if (a_dtype == 2 && Ab)
+    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, _Float16, 2, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                           reinterpret_cast<_Float16*>(Abar), reinterpret_cast<_Float16*>(Ab), part);
   else if (a_dtype == 2)
-    MP_TRAIN_RED_DISPATCH(k_train_out_act_bwd4, _Float16, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                          reinterpret_cast<_Float16*>(Abar), reinterpret_cast<_Float16*>(Ab), part);
+    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, _Float16, 1, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                           reinterpret_cast<_Float16*>(Abar), static_cast<_Float16*>(nullptr), part);
   else
-    MP_TRAIN_RED_DISPATCH(k_train_out_act_bwd4, float, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                          reinterpret_cast<float*>(Abar), static_cast<float*>(nullptr), part);
+    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, float, 1, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
+                           reinterpret_cast<float*>(Abar), static_cast<float*>(nullptr), part);
   MP_LAUNCHED("k_train_out_act_bwd4");
   return MP_OK;
 }
@@ -847,19 +927,27 @@ int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const f
 
 namespace mp {
 
-template <class T, class P>
-static void launch_act_bwd_part(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
-                                int64_t n, void* Abar, void* Ab, float* db_part) {
-  const P* pb = reinterpret_cast<const P*>(Pbar);
+template <class T, class PB, int P>
+static void launch_act_bwd_part_p(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
+                                  int64_t n, void* Abar, void* Ab, float* db_part) {
+  const PB* pb = reinterpret_cast<const PB*>(Pbar);
   T* a = reinterpret_cast<T*>(Abar);
   T* ab = reinterpret_cast<T*>(Ab);
   const dim3 g((unsigned)blocks), b(W / 4);
   switch (nt) {
-    case 0: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 0>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-    case 1: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 1>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-    case 2: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 2>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-    default: hipLaunchKernelGGL((k_train_act_bwd_part4<T, P, 3>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    case 0: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 0, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    case 1: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 1, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    case 2: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 2, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
+    default: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 3, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
   }
+}
+
+// with the blocked copy: two points per step (whole 16-byte chunks)
+template <class T, class PB>
+static void launch_act_bwd_part(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
+                                int64_t n, void* Abar, void* Ab, float* db_part) {
+  if (Ab && sizeof(T) == 2) launch_act_bwd_part_p<T, PB, 2>(nt, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part);
+  else launch_act_bwd_part_p<T, PB, 1>(nt, blocks, W, st, Z, Pbar, n, Abar, nullptr, db_part);
 }
 
 }  // namespace mp
