@@ -143,37 +143,59 @@ __global__ void k_band_mark_jobs(const BandJob* __restrict__ jobs, int n_jobs) {
   }
 }
 
-// Select each job's candidates for its level (wave ballot + one atomic per wave).
-__global__ void k_band_select_jobs(const BandJob* __restrict__ jobs, int n_jobs) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int lane = threadIdx.x & 63;
+// Select each job's candidates for its level: each thread classifies 8
+// candidates (strided by the block size), the block scans its threads' counts
+// and reserves its output range with one atomic (a per-wave atomic on the
+// job's one counter serialises).
+__global__ __launch_bounds__(256) void k_band_select_jobs(const BandJob* __restrict__ jobs, int n_jobs) {
+  constexpr int ITEMS = 8;
+  __shared__ uint32_t wsum[4], wbase;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int j = 0; j < n_jobs; ++j) {
     const BandJob& b = jobs[j];
     const int64_t n = *b.ccount;
-    // the loop bound is wave-uniform: every lane of a wave takes the ballots
-    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); w < n; w += stride) {
-      const int64_t i = w + lane;
-      bool flag = false;
-      uint32_t k = 0;
-      if (i < n) {
-        k = b.cand[i];
-        if (fabsf(b.vol[k]) < b.delta) {
-          const int64_t c = k / b.chunk;
-          bool alive = true;
-          for (int l = 1; l <= b.level; ++l) alive = alive && b.flags[(int64_t)(l - 1) * b.nchunks + c];
-          flag = alive;
+    // block-uniform loop: every thread reaches the barriers
+    for (int64_t b0 = (int64_t)blockIdx.x * 256 * ITEMS; b0 < n; b0 += (int64_t)gridDim.x * 256 * ITEMS) {
+      uint32_t keys[ITEMS];
+      uint32_t mask = 0;
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it) {
+        const int64_t i = b0 + it * 256 + threadIdx.x;
+        keys[it] = 0;
+        if (i < n) {
+          const uint32_t k = b.cand[i];
+          keys[it] = k;
+          if (fabsf(b.vol[k]) < b.delta) {
+            // grid indices are below 2^32: a 32-bit division (chunks past 2^32 hold everything)
+            const uint32_t c = k / (uint32_t)(b.chunk < 0xFFFFFFFFll ? b.chunk : 0xFFFFFFFFll);
+            bool alive = true;
+            for (int l = 1; l <= b.level; ++l) alive = alive && b.flags[(int64_t)(l - 1) * b.nchunks + c];
+            if (alive) mask |= 1u << it;
+          }
         }
       }
-      const uint64_t m = __ballot(flag);
-      if (m) {
-        uint32_t pos = 0;
-        if (lane == 0) pos = atomicAdd(b.lcount, (uint32_t)__popcll(m));
-        pos = __shfl(pos, 0);
-        if (flag) {
-          const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
-          b.list[pos + __popcll(below)] = k;
-        }
+      const uint32_t cnt = __popc(mask);
+      uint32_t incl = cnt;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = __shfl_up(incl, o);
+        if (lane >= o) incl += x;
       }
+      if (lane == 63) wsum[w] = incl;
+      __syncthreads();
+      uint32_t off = 0, tot = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        off += k < w ? wsum[k] : 0u;
+        tot += wsum[k];
+      }
+      if (threadIdx.x == 0) wbase = tot ? atomicAdd(b.lcount, tot) : 0u;
+      __syncthreads();
+      uint32_t pos = wbase + off + incl - cnt;
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it)
+        if (mask & (1u << it)) b.list[pos++] = keys[it];
+      __syncthreads();  // wsum / wbase are reused by the next iteration
     }
   }
 }

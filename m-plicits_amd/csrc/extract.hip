@@ -31,40 +31,69 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
 
 // any[c] = 1 if some candidate of chunk c has |s| < delta
 __global__ void k_band_mark(const uint32_t* cand, const uint32_t* ccount, const float* vol,
-                            float delta, int64_t chunk, uint8_t* any) {
+                            float delta, uint32_t chunk, uint8_t* any) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)*ccount) return;
   const uint32_t k = cand[i];
   if (fabsf(vol[k]) < delta) any[k / chunk] = 1;
 }
 
-// Select the candidates evaluated by level `level`.
-__global__ void k_band_select(const uint32_t* cand, const uint32_t* ccount, const float* vol,
-                              float delta, int64_t chunk, const uint8_t* any, int64_t nchunks,
-                              int level, uint32_t* list, uint32_t* lcount) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  bool flag = false;
-  uint32_t k = 0;
-  if (i < (int64_t)*ccount) {
-    k = cand[i];
-    if (fabsf(vol[k]) < delta) {
-      const int64_t c = k / chunk;
-      bool alive = true;
-      for (int j = 1; j <= level; ++j) alive = alive && any[(int64_t)(j - 1) * nchunks + c];
-      flag = alive;
+// Select the candidates evaluated by level `level`.  Each thread classifies
+// BAND_ITEMS candidates (strided by the block size, so the reads coalesce);
+// the block scans its threads' counts and reserves its output range with a
+// single atomic (one per BAND_ITEMS x 256 candidates: a per-wave atomic on the
+// one counter serialises).  Grid indices are below 2^32 (N <= 1625).
+constexpr int BAND_ITEMS = 8;
+
+__global__ __launch_bounds__(256) void k_band_select(const uint32_t* __restrict__ cand,
+                                                     const uint32_t* __restrict__ ccount,
+                                                     const float* __restrict__ vol, float delta, uint32_t chunk,
+                                                     const uint8_t* __restrict__ any, int64_t nchunks, int level,
+                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ lcount) {
+  __shared__ uint32_t wsum[4], wbase;
+  const int64_t n = (int64_t)*ccount;
+  const int64_t b0 = (int64_t)blockIdx.x * 256 * BAND_ITEMS;
+  if (b0 >= n) return;  // workgroup-uniform
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t keys[BAND_ITEMS];
+  uint32_t mask = 0;
+#pragma unroll
+  for (int it = 0; it < BAND_ITEMS; ++it) {
+    const int64_t i = b0 + it * 256 + threadIdx.x;
+    keys[it] = 0;
+    if (i < n) {
+      const uint32_t k = cand[i];
+      keys[it] = k;
+      if (fabsf(vol[k]) < delta) {
+        const uint32_t c = k / chunk;
+        bool alive = true;
+        for (int j = 1; j <= level; ++j) alive = alive && any[(int64_t)(j - 1) * nchunks + c];
+        if (alive) mask |= 1u << it;
+      }
     }
   }
-  const uint64_t m = __ballot(flag);
-  if (m) {
-    uint32_t pos = 0;
-    if (lane == 0) pos = atomicAdd(lcount, (uint32_t)__popcll(m));
-    pos = __shfl(pos, 0);
-    if (flag) {
-      const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
-      list[pos + __popcll(below)] = k;
-    }
+  // exclusive scan of the thread counts over the block
+  const uint32_t cnt = __popc(mask);
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t x = __shfl_up(incl, o);
+    if (lane >= o) incl += x;
   }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    off += k < w ? wsum[k] : 0u;
+    tot += wsum[k];
+  }
+  if (threadIdx.x == 0) wbase = tot ? atomicAdd(lcount, tot) : 0u;
+  __syncthreads();
+  uint32_t pos = wbase + off + incl - cnt;
+#pragma unroll
+  for (int it = 0; it < BAND_ITEMS; ++it)
+    if (mask & (1u << it)) list[pos++] = keys[it];
 }
 
 __global__ void k_grid_coords(int N, int mode, float h, int64_t first, int64_t n, float* xyz) {
@@ -152,6 +181,7 @@ int mp_extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const f
   float dmax = 0.f;
   for (int i = 0; i < n_deltas; ++i) dmax = deltas[i] > dmax ? deltas[i] : dmax;
   const int64_t nchunks = (n + chunk - 1) / chunk;
+  const int64_t chunk32 = chunk < n ? chunk : n;  // grid indices (< 2^32) divide by it in 32 bits
   // scratch: counters (256 B) | any flags [L-1][nchunks] | candidates [n] | list [n]
   const size_t off_any = 256;
   const size_t off_cand = (off_any + (size_t)(L - 1) * nchunks + 255) & ~(size_t)255;
@@ -190,10 +220,10 @@ int mp_extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const f
     int64_t nsel = 0;
     if (ncand > 0) {
       hipLaunchKernelGGL(k_band_mark, dim3(blocks_for(ncand, 256)), dim3(256), 0, s, cand,
-                         &counters[0], vol, d, chunk, any_l);
+                         &counters[0], vol, d, (uint32_t)chunk32, any_l);
       MP_LAUNCHED("k_band_mark");
-      hipLaunchKernelGGL(k_band_select, dim3(blocks_for(ncand, 256)), dim3(256), 0, s, cand,
-                         &counters[0], vol, d, chunk, any, nchunks, l, list, lcount);
+      hipLaunchKernelGGL(k_band_select, dim3(blocks_for(ncand, 256 * BAND_ITEMS)), dim3(256), 0, s, cand,
+                         &counters[0], vol, d, (uint32_t)chunk32, any, nchunks, l, list, lcount);
       MP_LAUNCHED("k_band_select");
       rc = read_count(lcount, &nsel, s);
       if (rc != MP_OK) return rc;
