@@ -34,6 +34,11 @@ int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t
 int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n, float* f, float* g,
                       int accumulate, hipStream_t stream, const unsigned long long* n_dev = nullptr);
 
+// The same in forward mode (four MFMA columns per point: value and three
+// tangents; level.hip), for widths whose reverse-mode kernel would spill.
+int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t n, float* f, float* g,
+                         int accumulate, hipStream_t stream, const unsigned long long* n_dev = nullptr);
+
 // Input gradient of sum_{l=lvl0}^{lvl1-1} f_l at n points (overwrites grad).
 int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
                 hipStream_t stream);

@@ -271,6 +271,12 @@ static int launch_grad_width(const LevelDev& L, const float* xyz, int64_t n, flo
 int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n, float* f, float* g,
                       int accumulate, hipStream_t stream, const unsigned long long* n_dev) {
   const LevelDev& L = st->lv[level].dev;
+  // forward mode where the reverse-mode kernel spills: 16-bit width 512 (and
+  // 256 with one hidden GEMM), FP32 widths >= 256
+  const int gemms = L.n_hidden - 1;
+  const bool fp32 = st->shape == SH_16x16x4_F32;
+  if (gemms >= 1 && (fp32 ? L.width >= 256 : (L.width == 512 || (L.width == 256 && gemms == 1))))
+    return launch_level_tangent(st->shape, L, xyz, n, f, g, accumulate, stream, n_dev);
   switch (st->shape) {
     case SH_32x32x16_BF16: return launch_grad_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, n_dev, stream);
     case SH_32x32x16_F16: return launch_grad_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, n_dev, stream);

@@ -62,6 +62,94 @@ static int launch_width(const LevelDev& L, int nc, const EvalTarget& t, hipStrea
   return MP_EUNSUPPORTED;
 }
 
+// f and the input gradient at n points, forward mode (LevelCore's tangent
+// columns: four columns per point), for the widths where the reverse-mode
+// gradient kernel does not fit its registers.  Lane group 0 stores: the value
+// column f, tangent column t component t - 1 of the gradient.
+template <class S, int W, int NC>
+__global__ __launch_bounds__(256) void k_level_tangent(LevelDev L, const float* __restrict__ xyz, int64_t n,
+                                                       float* f_out, float* g_out, int accumulate,
+                                                       const unsigned long long* n_dev) {
+  typedef LevelCore<S, W, NC, true, 4, true> M;
+  constexpr int PTS = M::TILE / 4;  // points per wave
+  extern __shared__ uint4 lds_raw[];
+  if (n_dev) n = (int64_t)*n_dev < n ? (int64_t)*n_dev : n;  // device count, n = upper bound
+  // workgroup-uniform exit: the waves of a workgroup share the stream's barriers
+  if ((int64_t)blockIdx.x * 4 * PTS >= n) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane % S::ROWS, tl = col & 3;
+  const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * PTS;
+  float x[NC][3];
+  int64_t idx[NC];
+  bool valid[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int64_t i = base + c * (S::ROWS / 4) + col / 4;
+    valid[c] = i < n;
+    idx[c] = valid[c] ? i : n - 1;
+    x[c][0] = xyz[3 * idx[c]];
+    x[c][1] = xyz[3 * idx[c] + 1];
+    x[c][2] = xyz[3 * idx[c] + 2];
+  }
+  typename S::Frag* lds = reinterpret_cast<typename S::Frag*>(lds_raw);
+  float f[NC];
+  M::eval(L, x, f, lds, lds + M::RING_FRAGS + wave * M::HANDOFF_FRAGS);
+  if (lane < S::ROWS) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (!valid[c]) continue;
+      float* o = tl == 0 ? (f_out ? f_out + idx[c] : nullptr) : g_out + 3 * idx[c] + (tl - 1);
+      if (o) *o = accumulate ? *o + f[c] : f[c];
+    }
+  }
+}
+
+template <class S, int W>
+static int launch_tangent_typed(const LevelDev& L, const float* xyz, int64_t n, float* f, float* g, int acc,
+                                const unsigned long long* n_dev, hipStream_t stream) {
+  constexpr int NC = default_cols<S>(W);
+  typedef LevelCore<S, W, NC, true, 4, true> M;
+  static_assert(M::LDS_BYTES <= 160 * 1024, "LDS budget of one workgroup");
+  const int64_t per_block = (int64_t)M::TILE;  // four waves x TILE / 4 points
+  const int64_t blocks = (n + per_block - 1) / per_block;
+  if (blocks <= 0) return MP_OK;
+  MP_REQUIRE(blocks < (1ll << 31), MP_EINVAL, "gradient launch: %lld points is too many", (long long)n);
+  static bool attr_set = false;
+  if (!attr_set) {
+    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level_tangent<S, W, NC>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)M::LDS_BYTES));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_level_tangent<S, W, NC>), dim3((unsigned)blocks), dim3(256), M::LDS_BYTES, stream, L, xyz, n,
+                     f, g, acc, n_dev);
+  MP_LAUNCHED("k_level_tangent");
+  return MP_OK;
+}
+
+template <class S>
+static int launch_tangent_width(const LevelDev& L, const float* xyz, int64_t n, float* f, float* g, int acc,
+                                const unsigned long long* n_dev, hipStream_t stream) {
+  switch (L.width) {
+    case 64: return launch_tangent_typed<S, 64>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 128: return launch_tangent_typed<S, 128>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 256: return launch_tangent_typed<S, 256>(L, xyz, n, f, g, acc, n_dev, stream);
+    case 512: return launch_tangent_typed<S, 512>(L, xyz, n, f, g, acc, n_dev, stream);
+  }
+  set_error("gradient launch: unsupported padded width %d", L.width);
+  return MP_EUNSUPPORTED;
+}
+
+int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t n, float* f, float* g,
+                         int accumulate, hipStream_t stream, const unsigned long long* n_dev) {
+  switch (shape) {
+    case SH_32x32x16_BF16: return launch_tangent_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, n_dev, stream);
+    case SH_32x32x16_F16: return launch_tangent_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, n_dev, stream);
+    case SH_16x16x4_F32: return launch_tangent_width<S16x16x4_f32>(L, xyz, n, f, g, accumulate, n_dev, stream);
+  }
+  set_error("gradient launch: unsupported fragment shape %d", shape);
+  return MP_EUNSUPPORTED;
+}
+
 int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream) {
   const LevelDev& L = st->lv[level].dev;
   switch (st->shape) {

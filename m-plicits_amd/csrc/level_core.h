@@ -38,7 +38,14 @@ typedef __attribute__((address_space(3))) void lds_void;
 // WAVES: waves of the workgroup that share the staged weight stream (four: one
 // per SIMD; eight: two per SIMD, for single-column tiles whose registers and
 // LDS hand-off regions fit twice).
-template <class S, int W, int NC, bool ALLOW_STAGED, int WAVES = 4>
+// TANGENT: forward-mode input gradient.  Columns 4 p .. 4 p + 3 of a tile
+// carry point p's value and its derivatives along x, y, z: the first layer
+// sees (x, y, z, 1) in the value column and (e_t, 0) in tangent column t, the
+// biases reach the value column only, and tangent column t activates to
+// cos(a_0) * a_t with a_0 taken from the value column of its quad (the
+// accumulators are in revolutions: 2 pi cos_rev(r_0) r_t).  The output of
+// tangent column t is df/dx_t.
+template <class S, int W, int NC, bool ALLOW_STAGED, int WAVES = 4, bool TANGENT = false>
 struct LevelCore {
   typedef typename S::Frag Frag;
   typedef typename S::Acc Acc;
@@ -153,15 +160,36 @@ struct LevelCore {
     return v + __shfl_xor(v, 32);
   }
 
+  // Tangent index of this lane's column (0: the value column).
+  static __device__ __forceinline__ int tangent_of(int lane) { return TANGENT ? (lane % ROWS) & 3 : 0; }
+
+  // Activation of an accumulator in revolutions: sin, or in a tangent column
+  // the derivative cos(r_0) 2 pi r_t (every lane must call it: it shuffles).
+  static __device__ __forceinline__ float act_rev(float v, int lane) {
+    if constexpr (TANGENT) {
+      const float r0 = __shfl(v, lane & ~3);
+      return tangent_of(lane) == 0 ? sin_rev(v) : cos_rev(r0) * (6.28318530717958648f * v);
+    } else {
+      return sin_rev(v);
+    }
+  }
+
   // Layer 0 into the B registers of layer 1.
   static __device__ __forceinline__ void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][KBS],
                                       int lane, int g) {
     if constexpr (MFMA_FIRST) {
       float bxy[NC], bz1[NC];
+      const int tl = tangent_of(lane);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        bxy[c] = g == 0 ? x[c][0] : x[c][1];
-        bz1[c] = g == 0 ? x[c][2] : 1.f;
+        if (TANGENT && tl != 0) {
+          // unit direction e_t, no bias input
+          bxy[c] = g == 0 ? (tl == 1 ? 1.f : 0.f) : (tl == 2 ? 1.f : 0.f);
+          bz1[c] = g == 0 ? (tl == 3 ? 1.f : 0.f) : 0.f;
+        } else {
+          bxy[c] = g == 0 ? x[c][0] : x[c][1];
+          bz1[c] = g == 0 ? x[c][2] : 1.f;
+        }
       }
       Acc zero;
 #pragma unroll
@@ -179,11 +207,12 @@ struct LevelCore {
         for (int q = 0; q < CREGS; ++q) {
           const int t = rb * CREGS + q;
 #pragma unroll
-          for (int c = 0; c < NC; ++c) S::set(b[c][t / EPL], t % EPL, sin_rev(acc[c][q]));
+          for (int c = 0; c < NC; ++c) S::set(b[c][t / EPL], t % EPL, act_rev(acc[c][q], lane));
         }
       }
     } else {
       // B element e of K block kb holds virtual unit v = KB kb + EPL g + e
+      const int tl = tangent_of(lane);
 #pragma unroll
       for (int kb = 0; kb < KBS; ++kb) {
 #pragma unroll
@@ -192,15 +221,23 @@ struct LevelCore {
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const float a = fmaf(w.z, x[c][2], fmaf(w.y, x[c][1], fmaf(w.x, x[c][0], w.w)));
-            S::set(b[c][kb], e, sin_reduced(a));
+            if (TANGENT && tl != 0) {
+              // a_0 from this lane's own point, a_t = w . e_t (radians)
+              const float at = tl == 1 ? w.x : (tl == 2 ? w.y : w.z);
+              S::set(b[c][kb], e, cos_reduced(a) * at);
+            } else {
+              S::set(b[c][kb], e, sin_reduced(a));
+            }
           }
         }
       }
     }
   }
 
-  // A level with one sine layer: f = w_L . sin(W_0 x + b_0) on the VALU.
+  // A level with one sine layer: f = w_L . sin(W_0 x + b_0) on the VALU
+  // (tangent columns: w_L . (cos(a) w_t), scaled to radians).
   static __device__ __forceinline__ void single(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC], int g) {
+    const int tl = tangent_of(threadIdx.x & 63);
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = 0.f;
     for (int v0 = 0; v0 < W; v0 += KB) {
@@ -212,7 +249,12 @@ struct LevelCore {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const float a = fmaf(w.z, x[c][2], fmaf(w.y, x[c][1], fmaf(w.x, x[c][0], w.w)));
-          f[c] = fmaf(wl, MFMA_FIRST ? sin_rev(a) : sin_reduced(a), f[c]);
+          if (TANGENT && tl != 0) {
+            const float at = tl == 1 ? w.x : (tl == 2 ? w.y : w.z);
+            f[c] = fmaf(wl, (MFMA_FIRST ? cos_rev(a) * 6.28318530717958648f : cos_reduced(a)) * at, f[c]);
+          } else {
+            f[c] = fmaf(wl, MFMA_FIRST ? sin_rev(a) : sin_reduced(a), f[c]);
+          }
         }
       }
     }
@@ -227,6 +269,12 @@ struct LevelCore {
     Acc acc[2][NC];
     Acc bias;
     loadc(bc + g * CREGS, bias);
+    // tangent columns carry no bias
+    const float bmask = (TANGENT && tangent_of(threadIdx.x & 63) != 0) ? 0.f : 1.f;
+    if constexpr (TANGENT) {
+#pragma unroll
+      for (int q = 0; q < CREGS; ++q) bias[q] *= bmask;
+    }
 #pragma unroll
     for (int rb = 0; rb < RBS; ++rb) {
       Acc bias_next;
@@ -243,7 +291,13 @@ struct LevelCore {
         }
         asm volatile("" ::: "memory");
       }
-      if (rb + 1 < RBS) bias = bias_next;
+      if (rb + 1 < RBS) {
+        bias = bias_next;
+        if constexpr (TANGENT) {
+#pragma unroll
+          for (int q = 0; q < CREGS; ++q) bias[q] *= bmask;
+        }
+      }
     }
 #pragma unroll
     for (int q = 0; q < CREGS; ++q) epi(RBS - 1, q, acc[(RBS - 1) & 1]);
@@ -260,10 +314,12 @@ struct LevelCore {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const float bl = *L.blp;
     const int n_gemm = L.n_hidden - 1;
+    // the output bias reaches the value column only
+    const float blv = tangent_of(lane) == 0 ? bl : 0.f;
     if (n_gemm == 0) {
       single(L, x, f, g);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
+      for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + blv;
       return;
     }
 #pragma unroll
@@ -281,7 +337,7 @@ struct LevelCore {
         const int t = rb * CREGS + q;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          S::set(st[c], t % EPL, sin_rev(acc[c][q]));
+          S::set(st[c], t % EPL, act_rev(acc[c][q], lane));
           if (t % EPL == EPL - 1) hand[(c * KBS + t / EPL) * 64 + lane] = st[c];
         }
       });
@@ -299,12 +355,12 @@ struct LevelCore {
          [&](int rb, int q, const Acc (&acc)[NC]) {
            if (q == 0 && rb + 1 < RBS) loadc(L.wlc + ((rb + 1) * GROUPS + g) * CREGS, wl_next);
 #pragma unroll
-           for (int c = 0; c < NC; ++c) f[c] = fmaf(wl[q], sin_rev(acc[c][q]), f[c]);
+           for (int c = 0; c < NC; ++c) f[c] = fmaf(wl[q], act_rev(acc[c][q], lane), f[c]);
            if (q == CREGS - 1 && rb + 1 < RBS) wl = wl_next;
          });
     A.finish();
 #pragma unroll
-    for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + bl;
+    for (int c = 0; c < NC; ++c) f[c] = group_sum(f[c]) + blv;
   }
 };
 
