@@ -360,7 +360,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the tracing and training figures")
-    ap.add_argument("--precisions", default="bf16,fp16,fp32",
+    ap.add_argument("--precisions", default="bf16,bf16x3,fp16,fp32",
                     help="comma list of paths to time (profiling runs use one)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB tensor (HBM counter calibration for rocprofv3 --pmc)")
@@ -383,7 +383,7 @@ def main():
     torch.cuda.set_device(0)
     levels = load_levels()
     if args.only:
-        precs = args.precisions.split(",") if args.precisions != "bf16,fp16,fp32" else None
+        precs = args.precisions.split(",") if args.precisions != "bf16,bf16x3,fp16,fp32" else None
         fig = {"trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"]),
                "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps),
                "train_data": lambda: bench_train_data(levels),
@@ -453,6 +453,8 @@ def main():
                    "grid": "reference", "global_batch": N ** 3, "parallelism": "none (1 GPU)"},
         "ms_by_precision": {p: round(r["ms"], 3) for p, r in results.items()},
         "speedup_bf16_vs_fp32": round(fp32["ms"] / b["ms"], 2) if fp32 else None,
+        "speedup_bf16x3_vs_fp32": (round(fp32["ms"] / results["bf16x3"]["ms"], 2)
+                                   if fp32 and "bf16x3" in results else None),
         "level_counts": b["counts"], "mesh": {"verts": b["verts"], "faces": b["faces"]},
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
         "trace_16x1080p": tr,

@@ -39,6 +39,13 @@ extern "C" {
 #define MP_FP32 0
 #define MP_BF16 1
 #define MP_FP16 2
+/* three-term BF16: every weight and activation as a BF16 pair hi + lo, each
+ * product as hi*hi + hi*lo + lo*hi on the BF16 matrix cores with FP32
+ * accumulation (about 16 significant bits per operand).  Levels wider than 256
+ * run FP16 (their operand pairs would not fit a wave's registers).
+ * Evaluation (mp_eval without gradients) and extraction only: the gradient,
+ * tracing, rendering and batched entry points return MP_EUNSUPPORTED. */
+#define MP_BF16X3 3
 
 /* sample coordinates of the extraction grid */
 #define MP_GRID_REFERENCE 0 /* i3d/meshing.py:42-63 as torch evaluates it (float32 true division) */

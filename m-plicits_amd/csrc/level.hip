@@ -62,6 +62,19 @@ static int launch_width(const LevelDev& L, int nc, const EvalTarget& t, hipStrea
   return MP_EUNSUPPORTED;
 }
 
+// Three-term BF16: one column block per wave (the hi and lo operand sets of a
+// 256-wide layer fill the registers and, with the staging ring, the 160 KB of
+// LDS of four waves).
+static int launch_split(const LevelDev& L, const EvalTarget& t, hipStream_t stream) {
+  switch (L.width) {
+    case 64: return launch_typed<S32x32x16_bf16x3, 64, 1>(L, t, stream);
+    case 128: return launch_typed<S32x32x16_bf16x3, 128, 1>(L, t, stream);
+    case 256: return launch_typed<S32x32x16_bf16x3, 256, 1>(L, t, stream);
+  }
+  set_error("level launch: three-term BF16 supports padded widths up to 256, got %d", L.width);
+  return MP_EUNSUPPORTED;
+}
+
 // f and the input gradient at n points, forward mode (LevelCore's tangent
 // columns: four columns per point), for the widths where the reverse-mode
 // gradient kernel does not fit its registers.  Lane group 0 stores: the value
@@ -152,12 +165,13 @@ int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t
 
 int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t stream) {
   const LevelDev& L = st->lv[level].dev;
-  switch (st->shape) {
+  switch (st->lv[level].shape) {
     case SH_32x32x16_BF16: return launch_width<S32x32x16_bf16>(L, st->nc, t, stream);
     case SH_32x32x16_F16: return launch_width<S32x32x16_f16>(L, st->nc, t, stream);
     case SH_16x16x4_F32: return launch_width<S16x16x4_f32>(L, st->nc, t, stream);
+    case SH_32x32x16_BF16X3: return launch_split(L, t, stream);
   }
-  set_error("level launch: unsupported fragment shape %d", st->shape);
+  set_error("level launch: unsupported fragment shape %d", st->lv[level].shape);
   return MP_EUNSUPPORTED;
 }
 

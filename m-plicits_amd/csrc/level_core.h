@@ -45,6 +45,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 // cos(a_0) * a_t with a_0 taken from the value column of its quad (the
 // accumulators are in revolutions: 2 pi cos_rev(r_0) r_t).  The output of
 // tangent column t is df/dx_t.
+// S::SPLIT (three-term BF16): every B fragment has a lo partner (registers
+// b[c][LO + kb], hand-off region after the hi fragments) and every A step two
+// fragments in the stream (hi, then lo); a step issues hi.hi, hi.lo and lo.hi.
 template <class S, int W, int NC, bool ALLOW_STAGED, int WAVES = 4, bool TANGENT = false>
 struct LevelCore {
   typedef typename S::Frag Frag;
@@ -54,17 +57,21 @@ struct LevelCore {
   static_assert(!MFMA_FIRST || ROWS == 32, "the MFMA first layer uses the 32x32 C layout");
   static constexpr int RBS = W / ROWS;           // row blocks per layer
   static constexpr int KBS = W / KB;             // K blocks per layer
-  static constexpr int STEPS = RBS * KBS;        // A fragments per layer
+  static constexpr int STEPS = RBS * KBS;        // MFMA steps per layer
+  static constexpr bool SPLIT = S::SPLIT;
+  static constexpr int BP = SPLIT ? 2 : 1;       // fragments per operand (hi, lo)
+  static constexpr int LO = SPLIT ? KBS : 0;     // offset of the lo B fragments
+  static constexpr int LSTEPS = STEPS * BP;      // A fragments per layer
   static constexpr int TILE = NC * ROWS;         // points per wave
-  static constexpr int HANDOFF_FRAGS = NC * KBS * 64;  // per wave, in fragments
+  static constexpr int HANDOFF_FRAGS = BP * NC * KBS * 64;  // per wave, in fragments
 
-  static constexpr bool STAGED = ALLOW_STAGED && sizeof(Frag) == 16 && STEPS % 32 == 0;
+  static constexpr bool STAGED = ALLOW_STAGED && sizeof(Frag) == 16 && LSTEPS % 32 == 0;
   static constexpr int SF = 16;  // fragments per stage
   static constexpr int D = 4;    // LDS read-ahead (steps)
   static constexpr int RING_FRAGS = STAGED ? 2 * SF * 64 : 0;
   static constexpr int PF_REG = sizeof(Frag) == 16 ? 8 : 32;
-  static constexpr int PF = STAGED ? D : (STEPS < PF_REG ? STEPS : PF_REG);
-  static_assert(STEPS % PF == 0, "the look-ahead ring must tile a layer");
+  static constexpr int PF = STAGED ? D : (LSTEPS < PF_REG ? LSTEPS : PF_REG);
+  static_assert(LSTEPS % PF == 0, "the look-ahead ring must tile a layer");
   static_assert((size_t)(STAGED ? 2 * SF : PF) * 64 * sizeof(Frag) <= APK_PAD,
                 "the stream reads past the padding");
   static_assert(SF % WAVES == 0, "every wave copies the same share of a stage");
@@ -116,21 +123,21 @@ struct LevelCore {
         for (int d = 0; d < D; ++d) pf[d] = ring[d * 64 + lane];
       }
     }
-    // Fragment of step s of hidden layer j; issues the look-ahead.
+    // Fragment s of hidden layer j's stream; issues the look-ahead.
     __device__ __forceinline__ Frag next(int j, int s) {
       if constexpr (STAGED) {
         const int t = s + D;  // step whose fragment is read now
         if (t % SF == 0) {
           // every wave is past its last read of the slot about to be refilled
           barrier();
-          copy_stage((int64_t)j * (STEPS / SF) + t / SF + 1, (t / SF + 1) % 2);
+          copy_stage((int64_t)j * (LSTEPS / SF) + t / SF + 1, (t / SF + 1) % 2);
         }
         const Frag a = pf[s % D];
         pf[s % D] = ring[((t / SF) % 2 * SF + t % SF) * 64 + lane];
         return a;
       } else {
         const Frag a = pf[s % PF];
-        pf[s % PF] = A[((size_t)j * STEPS + s + PF) * 64 + lane];
+        pf[s % PF] = A[((size_t)j * LSTEPS + s + PF) * 64 + lane];
         return a;
       }
     }
@@ -174,8 +181,14 @@ struct LevelCore {
     }
   }
 
+  // Element e of an operand: hi, and in SPLIT mode the residual into lo.
+  static __device__ __forceinline__ void set2(Frag& hi, Frag& lo, int e, float v) {
+    S::set(hi, e, v);
+    if constexpr (SPLIT) S::set(lo, e, v - (float)hi[e]);
+  }
+
   // Layer 0 into the B registers of layer 1.
-  static __device__ __forceinline__ void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][KBS],
+  static __device__ __forceinline__ void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][BP * KBS],
                                       int lane, int g) {
     if constexpr (MFMA_FIRST) {
       float bxy[NC], bz1[NC];
@@ -207,7 +220,7 @@ struct LevelCore {
         for (int q = 0; q < CREGS; ++q) {
           const int t = rb * CREGS + q;
 #pragma unroll
-          for (int c = 0; c < NC; ++c) S::set(b[c][t / EPL], t % EPL, act_rev(acc[c][q], lane));
+          for (int c = 0; c < NC; ++c) set2(b[c][t / EPL], b[c][LO + t / EPL], t % EPL, act_rev(acc[c][q], lane));
         }
       }
     } else {
@@ -265,7 +278,7 @@ struct LevelCore {
   // accumulator register q of row block rb.
   template <class Epi>
   static __device__ __forceinline__ void gemm(Stream& A, int j, const float* __restrict__ bc,
-                                     const Frag (&b)[NC][KBS], int g, Epi epi) {
+                                     const Frag (&b)[NC][BP * KBS], int g, Epi epi) {
     Acc acc[2][NC];
     Acc bias;
     loadc(bc + g * CREGS, bias);
@@ -280,10 +293,20 @@ struct LevelCore {
       Acc bias_next;
 #pragma unroll
       for (int kb = 0; kb < KBS; ++kb) {
-        const Frag a = A.next(j, rb * KBS + kb);
+        const Frag a = A.next(j, BP * (rb * KBS + kb));
+        if constexpr (SPLIT) {
+          const Frag al = A.next(j, BP * (rb * KBS + kb) + 1);
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-          acc[rb & 1][c] = S::mma(a, b[c][kb], kb == 0 ? bias : acc[rb & 1][c]);
+          for (int c = 0; c < NC; ++c) {
+            acc[rb & 1][c] = S::mma(a, b[c][kb], kb == 0 ? bias : acc[rb & 1][c]);
+            acc[rb & 1][c] = S::mma(a, b[c][LO + kb], acc[rb & 1][c]);
+            acc[rb & 1][c] = S::mma(al, b[c][kb], acc[rb & 1][c]);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            acc[rb & 1][c] = S::mma(a, b[c][kb], kb == 0 ? bias : acc[rb & 1][c]);
+        }
         if (kb == 0 && rb + 1 < RBS) loadc(bc + ((rb + 1) * GROUPS + g) * CREGS, bias_next);
         if (rb > 0 && kb % KPQ == 0) {
 #pragma unroll
@@ -324,7 +347,7 @@ struct LevelCore {
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = 0.f;
-    Frag b[NC][KBS];
+    Frag b[NC][BP * KBS];
     Stream A{reinterpret_cast<const Frag*>(L.apk), ring, lane, wave, {}};
     A.start();
     first(L, x, b, lane, g);
@@ -332,19 +355,25 @@ struct LevelCore {
 
     // hidden layers that feed another hidden layer: B registers -> LDS -> B registers
     for (int j = 0; j + 1 < n_gemm; ++j) {
-      Frag st[NC];
+      Frag st[NC][BP];
       gemm(A, j, L.biasc + (size_t)j * W, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
         const int t = rb * CREGS + q;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          S::set(st[c], t % EPL, act_rev(acc[c][q], lane));
-          if (t % EPL == EPL - 1) hand[(c * KBS + t / EPL) * 64 + lane] = st[c];
+          set2(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
+          if (t % EPL == EPL - 1) {
+            hand[(c * KBS + t / EPL) * 64 + lane] = st[c][0];
+            if constexpr (SPLIT) hand[((NC + c) * KBS + t / EPL) * 64 + lane] = st[c][1];
+          }
         }
       });
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int kb = 0; kb < KBS; ++kb) b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
+        for (int kb = 0; kb < KBS; ++kb) {
+          b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
+          if constexpr (SPLIT) b[c][LO + kb] = hand[((NC + c) * KBS + kb) * 64 + lane];
+        }
     }
 
     // last hidden layer fused with the output layer (output weights of row

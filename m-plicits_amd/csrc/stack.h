@@ -15,11 +15,12 @@ struct mp_stack {
     const float* biases = nullptr;   // borrowed
     void* buf = nullptr;          // packed arrays (one allocation)
     size_t buf_bytes = 0;
+    int shape = mp::SH_16x16x4_F32;  // fragment shape of this level's hidden GEMMs
     mp::LevelDev dev{};
   };
   int n_levels = 0;
   int precision = MP_FP32;
-  int shape = mp::SH_16x16x4_F32;
+  int shape = mp::SH_16x16x4_F32;  // of every level, except MP_BF16X3 stacks (per level)
   int nc = 1;  // point column blocks per wave
   Level lv[MP_MAX_LEVELS];
   void* scratch = nullptr;  // extraction / tracing work lists, grown on demand

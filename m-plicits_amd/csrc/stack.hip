@@ -122,6 +122,8 @@ __global__ void k_pack_small(PackArgs a) {
 // Hidden layer j (1..H-1): element ((rb * KBS + kb) * 64 + lane) * EPL + e.
 // transpose = 0: A = W_j / (2 pi) (forward, revolutions), rows = layer-j units.
 // transpose = 1: A = W_j^T (backward, radians), rows = layer-(j-1) units.
+// Three-term BF16, forward: fragment 2 (rb * KBS + kb) holds bf16(w), the
+// next one bf16(w - bf16(w)); the transposed copy holds bf16(w) only.
 __global__ void k_pack_hidden(PackArgs a, int j, int transpose) {
   const ShapeInfo s = shape_info(a.shape);
   const int64_t per_layer = (int64_t)a.width * a.width;
@@ -141,7 +143,15 @@ __global__ void k_pack_hidden(PackArgs a, int j, int transpose) {
   else w = (u < out_j && r < in_j) ? a.W[a.woff[j] + (int64_t)u * in_j + r] : 0.f;
   const int64_t o = (int64_t)(j - 1) * per_layer + i;
   void* dst = transpose ? a.apkt : a.apk;
+  if (a.shape == SH_32x32x16_BF16X3 && !transpose) {
+    const __bf16 hi = (__bf16)w;
+    const int64_t oh = ((int64_t)(j - 1) * per_layer * 2) + ((blk * 2) * 64 + lane) * s.epl + e;
+    reinterpret_cast<__bf16*>(dst)[oh] = hi;
+    reinterpret_cast<__bf16*>(dst)[oh + 64 * s.epl] = (__bf16)(w - (float)hi);
+    return;
+  }
   switch (a.shape) {
+    case SH_32x32x16_BF16X3:
     case SH_32x32x16_BF16: case SH_16x16x32_BF16: reinterpret_cast<__bf16*>(dst)[o] = (__bf16)w; break;
     case SH_32x32x16_F16: case SH_16x16x32_F16: reinterpret_cast<_Float16*>(dst)[o] = (_Float16)w; break;
     default: reinterpret_cast<float*>(dst)[o] = w; break;
@@ -152,7 +162,7 @@ static int pack_level(const mp_stack* st, const mp_stack::Level& L, hipStream_t 
   PackArgs a{};
   a.W = L.weights;
   a.B = L.biases;
-  a.shape = st->shape;
+  a.shape = L.shape;
   a.width = L.width;
   a.n_hidden = L.n_hidden;
   a.dims[0] = 3;
@@ -212,15 +222,15 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
   *out = nullptr;
   MP_REQUIRE(levels != nullptr && n_levels >= 1 && n_levels <= MP_MAX_LEVELS, MP_EINVAL,
              "mp_stack_create: n_levels must be in [1, %d], got %d", MP_MAX_LEVELS, n_levels);
-  MP_REQUIRE(precision == MP_FP32 || precision == MP_BF16 || precision == MP_FP16, MP_EINVAL,
-             "mp_stack_create: unknown precision %d", precision);
+  MP_REQUIRE(precision == MP_FP32 || precision == MP_BF16 || precision == MP_FP16 || precision == MP_BF16X3,
+             MP_EINVAL, "mp_stack_create: unknown precision %d", precision);
   mp_stack* st = new mp_stack();
   st->n_levels = n_levels;
   st->precision = precision;
   st->shape = precision == MP_BF16 ? SH_32x32x16_BF16
               : precision == MP_FP16 ? SH_32x32x16_F16
+              : precision == MP_BF16X3 ? SH_32x32x16_BF16X3
                                      : SH_16x16x4_F32;
-  const ShapeInfo si = shape_info(st->shape);
   int maxw = 0;
   for (int l = 0; l < n_levels; ++l) {
     const mp_level_desc& d = levels[l];
@@ -251,12 +261,16 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     }
     maxw = L.width > maxw ? L.width : maxw;
     L.n_hidden = d.n_hidden;
+    // three-term BF16 up to width 256; wider levels FP16
+    L.shape = (st->shape == SH_32x32x16_BF16X3 && L.width > 256) ? SH_32x32x16_F16 : st->shape;
+    const ShapeInfo si = shape_info(L.shape);
+    const size_t parts = L.shape == SH_32x32x16_BF16X3 ? 2 : 1;
     L.weights = d.weights;
     L.biases = d.biases;
     const size_t W = (size_t)L.width;
     const size_t n_gemm = (size_t)(L.n_hidden - 1);
     const size_t sz_w0v = W * sizeof(float4);
-    const size_t sz_apk = n_gemm * W * W * si.elem_bytes;
+    const size_t sz_apk = n_gemm * W * W * si.elem_bytes * parts;
     const size_t sz_bias = n_gemm * W * sizeof(float);
     const size_t sz_wl = W * sizeof(float);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };

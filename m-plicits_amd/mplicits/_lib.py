@@ -20,7 +20,8 @@ MP_EUNSUPPORTED = -4
 MP_FP32 = 0
 MP_BF16 = 1
 MP_FP16 = 2
-PRECISIONS = {"fp32": MP_FP32, "bf16": MP_BF16, "fp16": MP_FP16}
+MP_BF16X3 = 3  # three-term BF16 (hi.hi + hi.lo + lo.hi); evaluation and extraction only
+PRECISIONS = {"fp32": MP_FP32, "bf16": MP_BF16, "fp16": MP_FP16, "bf16x3": MP_BF16X3}
 
 MP_GRID_REFERENCE = 0
 MP_GRID_LATTICE = 1

@@ -12,7 +12,8 @@ from the reference, by design:
     synchronisations in total instead of two per chunk;
   * marching cubes runs on the GPU; a vertex inside a cube (tiling index 12)
     is created once per cube and shared by that cube's triangles;
-  * ``precision`` ("fp32", "bf16", "fp16") selects the hidden-GEMM arithmetic.
+  * ``precision`` ("fp32", "bf16", "fp16", "bf16x3") selects the hidden-GEMM
+    arithmetic.
 """
 import ctypes
 import time

@@ -56,7 +56,7 @@ class SIREN(nn.Module):
     by the kernels as ``sin(w * (W x + b))``; checkpoints saved after
     ``update_omegas(1)`` carry them baked into the weights.
 
-    Extra attribute ``precision`` ("fp32", "bf16" or "fp16") selects the
+    Extra attribute ``precision`` ("fp32", "bf16", "fp16" or "bf16x3") selects the
     arithmetic of the hidden-layer GEMMs on the GPU.
     """
 

@@ -39,7 +39,10 @@ class LevelStack:
     Parameters
     ----------
     models: sequence of SIREN modules (3 -> ... -> 1), one per level.
-    precision: "fp32", "bf16" or "fp16": arithmetic of the hidden GEMMs.
+    precision: "fp32", "bf16", "fp16" or "bf16x3": arithmetic of the hidden
+        GEMMs.  "bf16x3" holds every operand as a BF16 pair hi + lo and forms
+        hi.hi + hi.lo + lo.hi (levels wider than 256 run FP16); it serves
+        evaluation without gradients and extraction only.
     """
 
     def __init__(self, models, precision="fp32"):

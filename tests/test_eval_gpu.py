@@ -22,7 +22,7 @@ CASES = [
     ([256, 256, 256, 256, 256], 30.0, 1.0),
 ]
 # Max |f_gpu - f_fp64| relative to max |f_fp64| over the sample.
-REL_TOL = {"fp32": 2e-5, "fp16": 2e-3, "bf16": 1e-2}
+REL_TOL = {"fp32": 2e-5, "fp16": 2e-3, "bf16": 1e-2, "bf16x3": 2e-5}
 
 
 def _points(n, seed):
@@ -30,7 +30,7 @@ def _points(n, seed):
     return torch.rand(n, 3, generator=g, dtype=torch.float64) * 2 - 1
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16", "bf16x3"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
 def test_forward_matches_fp64_oracle(case, precision):
     hidden, omega, last = case
@@ -42,7 +42,9 @@ def test_forward_matches_fp64_oracle(case, precision):
     err = np.abs(f - ref).max()
     scale = np.abs(ref).max()
     print(f"{precision} {hidden}: max err {err:.3e}, max |f| {scale:.3e}")
-    assert err <= REL_TOL[precision] * scale
+    # three-term BF16 runs levels wider than 256 in FP16
+    tol = REL_TOL["fp16"] if precision == "bf16x3" and max(hidden) > 256 else REL_TOL[precision]
+    assert err <= tol * scale
 
 
 def test_fp32_absolute_error_on_unit_outputs():
@@ -53,6 +55,34 @@ def test_fp32_absolute_error_on_unit_outputs():
     f = LevelStack([m], "fp32").eval(x64.float().cuda()).double().cpu().numpy()
     assert np.abs(ref).max() > 0.3
     assert np.abs(f - ref).max() <= 1e-5
+
+
+def test_bf16x3_absolute_error_on_unit_outputs():
+    # the model of the FP32 test above (outputs up to ~3.5): three-term BF16
+    # within 5e-5 absolute, plain BF16 off by more than 1e-3
+    m = seeded_siren([256, 256, 256], 30.0, seed=99, last_scale=30.0).cuda()
+    x64 = _points(20000, 3)
+    ref = osiren.forward(osiren.layers_from_model(m), x64.numpy())
+    f3 = LevelStack([m], "bf16x3").eval(x64.float().cuda()).double().cpu().numpy()
+    f1 = LevelStack([m], "bf16").eval(x64.float().cuda()).double().cpu().numpy()
+    print(f"bf16x3 max err {np.abs(f3 - ref).max():.3e}, bf16 {np.abs(f1 - ref).max():.3e}")
+    assert np.abs(f3 - ref).max() <= 5e-5
+    assert np.abs(f1 - ref).max() > 1e-3
+
+
+def test_bf16x3_levels_mixed_with_fp16_and_no_gradients():
+    levels = [seeded_siren([256, 256, 256], 30.0, seed=7).cuda(),
+              seeded_siren([512, 512, 512], 90.0, seed=8, last_scale=0.1).cuda()]
+    x64 = _points(3001, 4)
+    st = LevelStack(levels, "bf16x3")
+    f = st.eval(x64.float().cuda()).double().cpu().numpy()
+    # level 1 (512 wide) runs FP16: the sum equals level 0 in three-term BF16
+    # plus level 1 in FP16, evaluated apart
+    f0 = LevelStack(levels[:1], "bf16x3").eval(x64.float().cuda()).double().cpu().numpy()
+    f1 = LevelStack(levels[1:], "fp16").eval(x64.float().cuda()).double().cpu().numpy()
+    assert np.abs(f - (f0 + f1)).max() <= 1e-6
+    with pytest.raises(NotImplementedError):
+        st.eval(x64.float().cuda(), grad=True)
 
 
 def test_level_sum_and_empty_input():
