@@ -84,3 +84,19 @@ def test_reduced_precision_volume_close():
         _, counts = st.extract_volume(N, [0.05, 0.02], chunk=chunk)
         for a, b in zip(counts, counts32):
             assert abs(a - b) <= 0.01 * b + 8, (prec, counts, counts32)
+
+
+def test_bf16x3_culled_volume_matches_oracle():
+    # Three-term BF16 keeps the culled volume within 1e-4 of the FP64 sampling
+    # loop (all three levels 128/256 wide: every level runs the split path)
+    levels = _levels(widths=(256, 128, 256))
+    N, chunk = 40, 4096
+    ref = _oracle_volume(levels, N, [0.05, 0.02], chunk)
+    vol, counts = LevelStack(levels, "bf16x3").extract_volume(N, [0.05, 0.02], chunk=chunk)
+    _, counts32 = LevelStack(levels, "fp32").extract_volume(N, [0.05, 0.02], chunk=chunk)
+    v = vol.double().cpu().numpy().reshape(-1)
+    # a sample within rounding of a band edge may be culled differently
+    same = np.abs(v - ref) <= 1e-4
+    assert same.mean() >= 0.9995, same.mean()
+    assert [abs(a - b) <= 2 for a, b in zip(counts, counts32)] == [True] * 3, (counts, counts32)
+    assert 0 < counts[2] < counts[1] < N ** 3
