@@ -251,6 +251,7 @@ int mp_render(mp_stack* st, const float* cams, int n_cams, const mp_render_param
   MP_REQUIRE(st != nullptr && cams != nullptr && prm != nullptr && rgba != nullptr, MP_EINVAL,
              "mp_render: NULL argument");
   MP_REQUIRE(n_cams >= 1 && prm->width >= 1 && prm->height >= 1, MP_EINVAL, "mp_render: empty image");
+  MP_REQUIRE(st->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_render: %s", "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
   MP_REQUIRE(st->n_levels <= 3, MP_EUNSUPPORTED, "mp_render: the renderer draws at most three levels, got %d",
              st->n_levels);
   for (int l = 0; l < 3; ++l)
@@ -320,9 +321,12 @@ int mp_render(mp_stack* st, const float* cams, int n_cams, const mp_render_param
     case SH_32x32x16_F16:
       rc = nc == 2 ? launch_render<S32x32x16_f16, 2>(a, s) : launch_render<S32x32x16_f16, 1>(a, s);
       break;
-    default:
+    case SH_16x16x4_F32:
       rc = launch_render<S16x16x4_f32, 1>(a, s);
       break;
+    default:
+      set_error("mp_render: unsupported fragment shape %d", st->shape);
+      return MP_EUNSUPPORTED;
   }
   if (rc != MP_OK) return rc;
   const unsigned grid = (unsigned)((total + 255) / 256);

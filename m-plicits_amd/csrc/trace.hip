@@ -355,6 +355,7 @@ extern "C" {
 int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params* prm, float* depth,
              float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats, void* stream) {
   MP_REQUIRE(st != nullptr && cams != nullptr && prm != nullptr, MP_EINVAL, "mp_trace: NULL argument");
+  MP_REQUIRE(st->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_trace: %s", "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
   MP_REQUIRE(n_cams >= 1 && prm->width >= 1 && prm->height >= 1, MP_EINVAL, "mp_trace: empty image");
   MP_REQUIRE(depth != nullptr && steps != nullptr && rgba != nullptr, MP_EINVAL,
              "mp_trace: depth, steps and rgba are required");
@@ -423,9 +424,12 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
       rc = nc == 2 ? launch_trace<S32x32x16_f16, 2>(blocks, lds, s, a)
                    : launch_trace<S32x32x16_f16, 1>(blocks, lds, s, a);
       break;
-    default:
+    case SH_16x16x4_F32:
       rc = launch_trace<S16x16x4_f32, 1>(blocks, lds, s, a);
       break;
+    default:
+      set_error("mp_trace: unsupported fragment shape %d", st->shape);
+      return MP_EUNSUPPORTED;
   }
   if (rc != MP_OK) return rc;
   hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, depth,
@@ -472,6 +476,7 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
     const mp_trace_params* prm = &J.params;
     MP_REQUIRE(J.stack != nullptr && J.cam != nullptr, MP_EINVAL, "mp_trace_batch_plan: job %d: NULL stack or camera", j);
     MP_REQUIRE(J.stack->shape == shape, MP_EINVAL, "mp_trace_batch_plan: job %d: every stack must have the same precision", j);
+    MP_REQUIRE(J.stack->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_trace_batch_plan: %s", "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
     MP_REQUIRE(J.depth != nullptr && J.steps != nullptr && J.rgba != nullptr, MP_EINVAL,
                "mp_trace_batch_plan: job %d: depth, steps and rgba are required", j);
     MP_REQUIRE(prm->width >= 1 && prm->height >= 1 && prm->max_steps >= 1 && prm->hit_eps > 0.f &&
@@ -576,7 +581,10 @@ int mp_trace_batch_run(mp_trace_batch* p, void* stream) {
   switch (p->shape) {
     case SH_32x32x16_BF16: rc = launch_trace_jobs<S32x32x16_bf16>(p->blocks, lds, s, jobs, p->n_jobs); break;
     case SH_32x32x16_F16: rc = launch_trace_jobs<S32x32x16_f16>(p->blocks, lds, s, jobs, p->n_jobs); break;
-    default: rc = launch_trace_jobs<S16x16x4_f32>(p->blocks, lds, s, jobs, p->n_jobs); break;
+    case SH_16x16x4_F32: rc = launch_trace_jobs<S16x16x4_f32>(p->blocks, lds, s, jobs, p->n_jobs); break;
+    default:
+      set_error("mp_trace_batch_run: unsupported fragment shape %d", p->shape);
+      return MP_EUNSUPPORTED;
   }
   if (rc != MP_OK) return rc;
   hipLaunchKernelGGL(k_collect_jobs, dim3(p->blocks * 4), dim3(256), 0, s, jobs, outs, p->n_jobs);

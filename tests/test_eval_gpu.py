@@ -85,6 +85,16 @@ def test_bf16x3_levels_mixed_with_fp16_and_no_gradients():
         st.eval(x64.float().cuda(), grad=True)
 
 
+def test_bf16x3_refused_by_tracing_and_rendering():
+    from mplicits import tracer
+    st = LevelStack([seeded_siren([128, 128], 30.0, seed=3).cuda()], "bf16x3")
+    cam = [tracer.renderer_camera(32, 16)]
+    with pytest.raises(NotImplementedError, match="MP_BF16X3"):
+        tracer.trace(st, cam, 32, 16)
+    with pytest.raises(NotImplementedError, match="MP_BF16X3"):
+        tracer.render(st, cam, 32, 16)
+
+
 def test_level_sum_and_empty_input():
     levels = [seeded_siren([128, 128], 30.0, seed=s).cuda() for s in (1, 2, 3)]
     st = LevelStack(levels, "fp32")
