@@ -9,11 +9,13 @@ last layers x 0.1.  One step = band-culled evaluation of the 512^3
 reference-warped grid in 32^3 chunks (create_mesh_multistage semantics) plus
 GPU marching cubes of the volume (vertex/face emission), inputs resident in HBM.
 
-Prints one JSON line: value = ms per extraction of the BF16-MFMA path; the FP32
-and FP16 paths, band counts, SDF errors against the FP64 oracle, the roofline
-of the dominant kernel (coarse level over the grid) and the CPU baseline (the
-oracle's torch-CPU restatement of the reference loop plus the C marching cubes,
-on a bounded sample, scaled) ride along.
+Prints one JSON line: value = ms per extraction of the BF16-MFMA path; the FP32,
+FP16 and three-term BF16 paths, band counts, the roofline of the dominant kernel
+(coarse level over the grid) and the CPU leg ride along.  The CPU leg (skipped
+with --no-cpu-baseline) is the only part that touches the oracle: it times the
+oracle's torch-CPU restatement of the reference loop on a bounded sample
+(cpu_baseline) and uses the FP64 oracle as the checker of the GPU results (SDF
+errors of the volumes, SDF and normal errors at traced hits).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 """
@@ -99,7 +101,7 @@ def dominant_kernel_ms(model, precision, reps=5):
 
 
 def sdf_errors(levels, vol, n_surface=6, n_random=4):
-    """|GPU - FP64 oracle| over whole chunks (the oracle evaluates the same
+    """CPU leg: |GPU - FP64 oracle| over whole chunks (the oracle evaluates the same
     culling loop).  Samples whose band membership flips between the two are
     counted separately (their difference is a whole residual)."""
     from oracle import extract as oextract
@@ -154,11 +156,41 @@ def cpu_baseline(levels, budget_chunks=96):
     return t_eval, ids, threads
 
 
-def bench_trace(levels, precisions, reps=2):
+def trace_errors(levels, cams, width, height, r, n=2048, seed=0):
+    """CPU leg: FP64 oracle at n traced hits of r (hit = eye + depth * d with
+    the oracle's ray directions): |sum of the levels| there, and the angle
+    between the returned normal and the FP64 gradient of the sum."""
+    from oracle import siren as osiren
+    from oracle import trace as otrace
+    depth = r["depth"].reshape(-1)
+    hits = torch.nonzero(depth >= 0).flatten()
+    if len(hits) == 0:
+        return None
+    pick = hits[torch.randperm(len(hits), generator=torch.Generator().manual_seed(seed))[:n].to(hits.device)]
+    ids = pick.cpu().numpy()
+    d = depth[pick].double().cpu().numpy()
+    nrm = r["normals"].reshape(-1, 3)[pick].double().cpu().numpy()
+    P, D, t0, _ = otrace.rays(cams, width, height, ids=ids)
+    X = (P - t0[:, None] * D) + d[:, None] * D
+    f = np.zeros(len(ids))
+    g = np.zeros((len(ids), 3))
+    for m in levels:
+        fl, gl = osiren.forward_grad(osiren.layers_from_model(m), X)
+        f += fl.reshape(-1)
+        g += gl
+    cosang = (nrm * g).sum(1) / (np.linalg.norm(nrm, axis=1) * np.linalg.norm(g, axis=1))
+    ang = np.degrees(np.arccos(np.clip(cosang, -1.0, 1.0)))
+    return {"hit_sdf_abs_max": float(np.abs(f).max()), "hit_sdf_abs_mean": float(np.abs(f).mean()),
+            "normal_deg_max": float(ang.max()), "normal_deg_mean": float(ang.mean()), "hits_checked": len(ids)}
+
+
+def bench_trace(levels, precisions, reps=2, check=False):
     """16 seeded orbit cameras at 1920x1080 (one persistent launch + normals +
     shading per frame set), through the coarse model alone and through the
-    three-level stack (bands 0.05, 0.02): rays per second."""
-    cams = camera_block(orbit_cameras(16, seed=MASTER_SEED + 7))
+    three-level stack (bands 0.05, 0.02): rays per second; with check (CPU
+    leg) the FP64 errors at the hits of the last frame set."""
+    cam_list = orbit_cameras(16, seed=MASTER_SEED + 7)
+    cams = camera_block(cam_list)
     out = {}
     for name, lv, deltas in (("coarse", levels[:1], []), ("stack3", levels, list(DELTAS))):
         for prec in precisions:
@@ -173,6 +205,9 @@ def bench_trace(levels, precisions, reps=2):
             key = prec if name == "coarse" else f"{name}_{prec}"
             out[key] = {"rays_per_s": round(16 * 1920 * 1080 / dt, 1), "ms": round(dt * 1e3, 2),
                         "hits": r["hits"], "evals": r["evals"]}
+            if check:
+                out[key]["vs_fp64"] = trace_errors(lv, cam_list, 1920, 1080, r)
+            del r
     return out
 
 
@@ -384,7 +419,7 @@ def main():
     levels = load_levels()
     if args.only:
         precs = args.precisions.split(",") if args.precisions != "bf16,bf16x3,fp16,fp32" else None
-        fig = {"trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"]),
+        fig = {"trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"], check=not args.no_cpu_baseline),
                "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps),
                "train_data": lambda: bench_train_data(levels),
                "render": lambda: bench_render(tuple(precs or ("fp16", "fp32")), frames=args.steps * 20),
@@ -398,13 +433,13 @@ def main():
         st = LevelStack(levels, prec)
         steps = args.steps if prec != "fp32" else max(1, min(args.steps, 2))
         ms, counts, nv, nf = timed(st, vol, steps, min(args.warmup, 1) if prec == "fp32" else args.warmup)
-        err = sdf_errors(levels, vol)
+        err = sdf_errors(levels, vol) if not args.no_cpu_baseline else None
         results[prec] = {"ms": ms, "counts": counts, "verts": nv, "faces": nf, "sdf_err": err}
         print(f"# {prec}: {ms:.2f} ms/extraction, counts {counts}, mesh {nv} v / {nf} f, err {err}",
               file=sys.stderr, flush=True)
         del st
 
-    tr = bench_trace(levels, ["bf16", "fp32"]) if not args.no_extras else None
+    tr = bench_trace(levels, ["bf16", "fp16", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
     if tr:
         print(f"# trace: {tr}", file=sys.stderr, flush=True)
     tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
