@@ -81,6 +81,27 @@ def timed(stack, vol, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps, counts, nv, nf
 
 
+def mesh_of(vol):
+    """Vertices (world units: the reference's spacing 2 / (N - 1), origin -1)
+    and MC normals of the volume's zero level set, on the host."""
+    v, _, nrm, _ = marching_cubes_gpu(vol, 0.0)
+    return v.double().cpu().numpy() * (2.0 / (N - 1)) - 1.0, nrm.double().cpu().numpy()
+
+
+def mesh_distance(a, b):
+    """Symmetric nearest-vertex distances between meshes a and b (chamfer =
+    mean of the two directed means, hausdorff = max) and the angle between the
+    MC normals of matched vertices."""
+    from scipy.spatial import cKDTree
+    da, ia = cKDTree(b[0]).query(a[0])
+    db, _ = cKDTree(a[0]).query(b[0])
+    na, nb = a[1], b[1][ia]
+    c = (na * nb).sum(1) / np.maximum(np.linalg.norm(na, axis=1) * np.linalg.norm(nb, axis=1), 1e-30)
+    ang = np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
+    return {"chamfer": float((da.mean() + db.mean()) / 2), "hausdorff": float(max(da.max(), db.max())),
+            "normal_deg_mean": float(ang.mean()), "normal_deg_max": float(ang.max()), "voxel": 2.0 / (N - 1)}
+
+
 def dominant_kernel_ms(model, precision, reps=5):
     """Average duration of the coarse-level grid kernel: a one-level unculled
     extraction is exactly one launch of it; HIP events on the current stream."""
@@ -434,7 +455,7 @@ def main():
         steps = args.steps if prec != "fp32" else max(1, min(args.steps, 2))
         ms, counts, nv, nf = timed(st, vol, steps, min(args.warmup, 1) if prec == "fp32" else args.warmup)
         err = sdf_errors(levels, vol) if not args.no_cpu_baseline else None
-        results[prec] = {"ms": ms, "counts": counts, "verts": nv, "faces": nf, "sdf_err": err}
+        results[prec] = {"ms": ms, "counts": counts, "verts": nv, "faces": nf, "sdf_err": err, "mesh": mesh_of(vol)}
         print(f"# {prec}: {ms:.2f} ms/extraction, counts {counts}, mesh {nv} v / {nf} f, err {err}",
               file=sys.stderr, flush=True)
         del st
@@ -492,6 +513,8 @@ def main():
                                    if fp32 and "bf16x3" in results else None),
         "level_counts": b["counts"], "mesh": {"verts": b["verts"], "faces": b["faces"]},
         "sdf_err_vs_fp64": {p: results[p]["sdf_err"] for p in results},
+        "mesh_vs_fp32": ({p: mesh_distance(results[p]["mesh"], fp32["mesh"]) for p in results if p != "fp32"}
+                         if fp32 else None),
         "trace_16x1080p": tr,
         "train_step_2e18": tn,
         "train_data_512_mesh": td,
