@@ -372,7 +372,24 @@ struct Mlp {
 // ---------------------------------------------------------------------------
 __device__ inline void grid_coord(int64_t idx, int N, int mode, float h, float out[3]) {
   float vx, vy, vz;
-  if (mode == 0) {
+  if ((N & (N - 1)) == 0 && N <= 1024) {
+    // N = 2^k: dividing by N is an exact scaling and, for 0 <= a < N^2 (so
+    // ulp(a) < N), fmod(a, N) = a - floor(a / N) N is exact as one fma: the
+    // same values as the general expressions below, without 64-bit divisions
+    // or fmodf
+    const int k = __ffs(N) - 1;
+    const float fN = (float)N, inv = 1.0f / fN;
+    vz = (float)(int)(idx & (N - 1));
+    if (mode == 0) {
+      const float t = (float)idx * inv;
+      vy = fmaf(-floorf(t * inv), fN, t);
+      const float u = t * inv;
+      vx = fmaf(-floorf(u * inv), fN, u);
+    } else {
+      vy = (float)(int)((idx >> k) & (N - 1));
+      vx = (float)(idx >> (2 * k));
+    }
+  } else if (mode == 0) {
     const float fN = (float)N;
     vz = (float)(idx % N);
     const float t = __fdiv_rn((float)idx, fN);

@@ -128,6 +128,25 @@ def test_grid_coords_bit_exact(N, mode):
     np.testing.assert_array_equal(xyz, ogrid.coords(N, np.arange(first, total), mode))
 
 
+@pytest.mark.parametrize("mode", ["reference", "lattice"])
+@pytest.mark.parametrize("N", [2, 8, 64, 256, 512, 1024])
+def test_grid_coords_power_of_two_ranges_bit_exact(N, mode):
+    # power-of-two N takes the exact-scaling path: ranges at the start, around
+    # the float32 integer limits 2^24 .. 2^29 (where float32(i) rounds), at
+    # random places and at the end of the grid
+    total = N ** 3
+    starts = {0, max(0, total - 40000)}
+    for e in range(24, 30):
+        if 2 ** e < total:
+            starts.add(max(0, 2 ** e - 20000))
+    rng = np.random.default_rng(N)
+    starts.update(int(v) for v in rng.integers(0, max(1, total - 40000), 4))
+    for first in sorted(starts):
+        n = min(40000, total - first)
+        xyz = grid_coords(N, mode, first=first, n=n).cpu().numpy()
+        np.testing.assert_array_equal(xyz, ogrid.coords(N, np.arange(first, first + n), mode), err_msg=str(first))
+
+
 GRAD_TOL = {"fp32": 2e-5, "fp16": 3e-3, "bf16": 2e-2}
 
 
