@@ -187,9 +187,25 @@ struct LevelCore {
     if constexpr (SPLIT) S::set(lo, e, v - (float)hi[e]);
   }
 
+  // Layer-0 operands of the FP32 matrix cores (16-bit shapes), loaded by the
+  // caller ahead of its own set-up (point coordinates) so that their latency
+  // overlaps it.
+  struct First {
+    float2 w[MFMA_FIRST ? RBS : 1];
+  };
+  static __device__ __forceinline__ First load_first(const LevelDev& L) {
+    First p;
+    if constexpr (MFMA_FIRST) {
+      const int lane = threadIdx.x & 63;
+#pragma unroll
+      for (int rb = 0; rb < RBS; ++rb) p.w[rb] = L.w0m[rb * 64 + lane];
+    }
+    return p;
+  }
+
   // Layer 0 into the B registers of layer 1.
   static __device__ __forceinline__ void first(const LevelDev& L, const float (&x)[NC][3], Frag (&b)[NC][BP * KBS],
-                                      int lane, int g) {
+                                      int lane, int g, const First& w0) {
     if constexpr (MFMA_FIRST) {
       float bxy[NC], bz1[NC];
       const int tl = tangent_of(lane);
@@ -209,7 +225,7 @@ struct LevelCore {
       for (int q = 0; q < CREGS; ++q) zero[q] = 0.f;
 #pragma unroll
       for (int rb = 0; rb < RBS; ++rb) {
-        const float2 a = L.w0m[rb * 64 + lane];
+        const float2 a = w0.w[rb];
         Acc acc[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -332,6 +348,11 @@ struct LevelCore {
   // wave's handoff region of HANDOFF_FRAGS fragments.
   static __device__ __forceinline__ void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
                                      Frag* ring, Frag* hand) {
+    eval(L, x, f, ring, hand, load_first(L));
+  }
+  // The same with the layer-0 operands already loaded (load_first).
+  static __device__ __forceinline__ void eval(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
+                                     Frag* ring, Frag* hand, const First& w0) {
     const int lane = threadIdx.x & 63;
     const int g = lane / ROWS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -350,7 +371,7 @@ struct LevelCore {
     Frag b[NC][BP * KBS];
     Stream A{reinterpret_cast<const Frag*>(L.apk), ring, lane, wave, {}};
     A.start();
-    first(L, x, b, lane, g);
+    first(L, x, b, lane, g, w0);
     A.begin();
 
     // hidden layers that feed another hidden layer: B registers -> LDS -> B registers
@@ -412,6 +433,9 @@ __device__ __forceinline__ void eval_block(const LevelDev& L, const EvalTarget& 
   typedef LevelCore<S, W, NC, true, WAVES> M;
   const int lane = threadIdx.x & 63;
   const int64_t base = (block * WAVES + (threadIdx.x >> 6)) * M::TILE;
+  // layer-0 operands first: their latency overlaps the coordinates below
+  const typename M::First w0 = M::load_first(L);
+  __builtin_amdgcn_sched_barrier(0);
 
   float x[NC][3];
   int64_t key[NC];
@@ -432,7 +456,7 @@ __device__ __forceinline__ void eval_block(const LevelDev& L, const EvalTarget& 
     }
   }
   float f[NC];
-  M::eval(L, x, f, lds, lds + M::RING_FRAGS + (threadIdx.x >> 6) * M::HANDOFF_FRAGS);
+  M::eval(L, x, f, lds, lds + M::RING_FRAGS + (threadIdx.x >> 6) * M::HANDOFF_FRAGS, w0);
 
   // Lanes of group 0 own the store (every group holds the same value).
 #pragma unroll
