@@ -412,10 +412,11 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, int64_t n, co
   }
   int ea, eb, ta, tb;
   block_scan2(p.nv, 0, &ea, &eb, &ta, &tb);
-  if (i >= n) return;
+  if (i >= n || p.nv == 0) return;
+  // only points that own vertices are ever looked up (an edge a triangle uses
+  // has its owner's flag set; a centre vertex belongs to this point)
   const uint32_t base = block_off[2 * blockIdx.x] + (uint32_t)ea;
   vbase[i] = base;
-  if (p.nv == 0) return;
   const double v0 = mc_value(V, x, y, z);
   uint32_t id = base;
   for (int ax = 0; ax < 3; ++ax) {
