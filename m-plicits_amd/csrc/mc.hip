@@ -245,16 +245,22 @@ __device__ inline void mc_xyz(const McVol& V, int& x, int& y, int& z) {
 
 __device__ inline McPoint mc_point(const McVol& V, int x, int y, int z) {
   McPoint r{0, 0, 0};
-  r.flags = mc_edge_flags(V, x, y, z);
-  r.nv = __popc(r.flags);
   if (x + 1 < V.nx && y + 1 < V.ny && z + 1 < V.nz) {
     double v[8], raw[8];
     const int idx = mc_load_cell(V, x, y, z, v, raw);
+    // the owned edges' far ends are cell corners 1 (x), 3 (y) and 4 (z)
+    const int p0 = idx & 1;
+    r.flags = (((idx >> 1) & 1) != p0) | ((((idx >> 3) & 1) != p0) << 1) | ((((idx >> 4) & 1) != p0) << 2);
+    r.nv = __popc(r.flags);
     if (idx != 0 && idx != 255) {
       const int8_t* t;
       r.nt = mc_tiling(v, idx, &t);
       if (mc_uses_centre(t, r.nt)) r.nv += 1;
     }
+  } else {
+    // border point: owned edges only
+    r.flags = mc_edge_flags(V, x, y, z);
+    r.nv = __popc(r.flags);
   }
   return r;
 }
