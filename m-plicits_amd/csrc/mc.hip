@@ -406,9 +406,19 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_scan_apply(uint32_t* __restrict
   }
 }
 
+// Entries k (0 vertices, 1 faces) of block b's output range [begin, end):
+// exclusive block offsets, the grand totals after the last block.
+__device__ inline bool mc_block_empty(const uint32_t* block_off, const unsigned long long* tot, int k) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t end = b + 1 < gridDim.x ? block_off[2 * (b + 1) + k] : (uint32_t)tot[k];
+  return block_off[2 * b + k] == end;
+}
+
 __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, int64_t n, const uint32_t* block_off,
-                                                          uint32_t* vbase, float* verts, float* normals,
-                                                          float* values) {
+                                                          const unsigned long long* tot, uint32_t* vbase,
+                                                          float* verts, float* normals, float* values) {
+  // most blocks hold no surface: the whole workgroup leaves at once
+  if (mc_block_empty(block_off, tot, 0)) return;
   const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
   McPoint p{0, 0, 0};
   int x = 0, y = 0, z = 0;
@@ -504,7 +514,9 @@ __device__ inline uint32_t mc_vertex_id(const McVol& V, const uint32_t* vbase, i
 }
 
 __global__ __launch_bounds__(MC_BLOCK) void k_mc_faces(McVol V, int64_t n, const uint32_t* block_off,
-                                                       const uint32_t* vbase, int32_t* faces) {
+                                                       const unsigned long long* tot, const uint32_t* vbase,
+                                                       int32_t* faces) {
+  if (mc_block_empty(block_off, tot, 1)) return;
   const int64_t i = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x;
   int nt = 0, idx = 0;
   double v[8], raw[8];
@@ -597,12 +609,13 @@ int mp_mc_emit(const float* vol, int nz, int ny, int nx, float level, float* ver
   McVol V{vol, nx, ny, nz, (double)level};
   char* w = reinterpret_cast<char*>(ws);
   const size_t nb = mc_blocks(n);
+  const unsigned long long* tot = reinterpret_cast<const unsigned long long*>(w);
   const uint32_t* block = reinterpret_cast<const uint32_t*>(w + 256);
   uint32_t* vbase = reinterpret_cast<uint32_t*>(w + mc_vbase_off(n));
-  hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, vbase,
+  hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, tot, vbase,
                      verts, normals, values);
   MP_LAUNCHED("k_mc_vertices");
-  hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, vbase, faces);
+  hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(MC_BLOCK), 0, s, V, n, block, tot, vbase, faces);
   MP_LAUNCHED("k_mc_faces");
   return MP_OK;
 }
