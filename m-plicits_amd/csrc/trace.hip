@@ -110,14 +110,18 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
     }
     // one call site for every level keeps a single copy of each width's code
     for (int l = 0; l < a.n_levels; ++l) {
+      // column blocks with a ray that needs level l
+      unsigned cols = ~0u;
       if (l > 0) {
-        bool any = false;
+        cols = 0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) any |= need[c];
-        if (__ballot(any) == 0) break;
+        for (int c = 0; c < NC; ++c) cols |= (__ballot(need[c]) != 0 ? 1u : 0u) << c;
+        if (cols == 0) break;
       }
       float r[NC];
-      eval_level_any<S, NC>(a.levels[l], x, r, hand);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) r[c] = 0.f;
+      eval_level_any<S, NC>(a.levels[l], x, r, hand, cols);
       if (l == 0) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -410,7 +414,9 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   int dev = 0, cus = 0;
   MP_HIP(hipGetDevice(&dev));
   MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int nc = (st->nc == 2 && maxw <= 256) ? 2 : 1;
+  // two column blocks per wave unless the stack asks for one (512-wide levels
+  // then run one block at a time, only where a ray needs them)
+  const int nc = st->nc == 2 ? 2 : 1;
   // persistent grid: one workgroup of four waves per CU (the LDS handoff
   // regions take 128 KB of the CU's LDS)
   const int blocks = cus;

@@ -14,15 +14,29 @@ namespace mp {
 constexpr size_t TRACE_HAND_BYTES = 32 * 1024;
 
 // f_level at the NC points of this lane; hand: the wave's handoff region.
+// A 512-wide level runs one column block at a time (its operands fill the
+// registers for one), and only the blocks whose bit is set in cols (a
+// wave-uniform mask; the others keep f unchanged).
 template <class S, int NC>
 __device__ __forceinline__ void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
-                                      typename S::Frag* hand) {
+                                      typename S::Frag* hand, unsigned cols = ~0u) {
   switch (L.width) {
     case 64: LevelCore<S, 64, NC, false>::eval(L, x, f, nullptr, hand); break;
     case 128: LevelCore<S, 128, NC, false>::eval(L, x, f, nullptr, hand); break;
     case 256: LevelCore<S, 256, NC, false>::eval(L, x, f, nullptr, hand); break;
     default:
-      if constexpr (NC == 1) LevelCore<S, 512, 1, false>::eval(L, x, f, nullptr, hand);
+      if constexpr (NC == 1) {
+        LevelCore<S, 512, 1, false>::eval(L, x, f, nullptr, hand);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          if (!((cols >> c) & 1u)) continue;
+          const float xc[1][3] = {{x[c][0], x[c][1], x[c][2]}};
+          float fc[1];
+          LevelCore<S, 512, 1, false>::eval(L, xc, fc, nullptr, hand);
+          f[c] = fc[0];
+        }
+      }
       break;
   }
 }
