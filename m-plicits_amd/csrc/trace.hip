@@ -8,7 +8,11 @@
 // hit or left the scene are written out and their slots refilled from a global
 // ray counter, so the matrix cores only ever see live rays.  The coarse level
 // is evaluated for every live ray; a residual level only when some ray of the
-// tile is inside the band of the partial sum before it.
+// tile is inside the band of the partial sum before it.  A stack with residual
+// levels is traced in two launches: every ray on the coarse level until it
+// enters the coarse band, then the rays parked there with every level, so that
+// far-field rays (coarse only) and near-surface rays (every level) do not
+// share tiles.
 //
 // Step rule (the renderer's, with a hit test):
 //   F_0 = f_0(p).  Outside the coarse band (F_0 >= s * delta_0, s = band_scale,
@@ -47,12 +51,20 @@ template <class S, int NC>
 __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag* hand) {
   const int lane = threadIdx.x & 63;
   const int col = lane % S::ROWS;
-  const int64_t total = a.total_rays;
+  // the band phase's queue is the list the coarse phase parked
+  const int64_t total = a.phase == TRACE_BAND ? (int64_t)*a.n_parked : a.total_rays;
+  const int n_eval = a.phase == TRACE_COARSE ? 1 : a.n_levels;
   int64_t rid[NC];
   float p[NC][3], d[NC][3], t[NC], t0[NC];
   int steps[NC];
+  // a parked ray's first coarse evaluation repeats the one it was parked at:
+  // not counted again
+  bool fresh[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) rid[c] = -1;
+  for (int c = 0; c < NC; ++c) {
+    rid[c] = -1;
+    fresh[c] = false;
+  }
   const float R2 = a.bound_radius * a.bound_radius * 1.0001f;
   unsigned long long evals[MP_MAX_LEVELS] = {0, 0, 0, 0};
 
@@ -69,18 +81,34 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
       if (lane < S::ROWS && empty) {
         const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
         int64_t nid = (int64_t)(base + __popcll(below));
-        while (nid < total) {
-          bool inside;
-          ray_init(a, nid, p[c], d[c], t0[c], inside);
-          if (inside) break;
-          // the ray never meets the bounding sphere: a miss, then the next ray
-          a.depth[nid] = -1.0f;
-          a.steps[nid] = 0;
-          nid = (int64_t)atomicAdd(a.next_ray, 1ull);
+        if (a.phase == TRACE_BAND) {
+          if (nid < total) {
+            const ParkedRay& r = a.park[nid];
+            const int64_t id = r.rid;
+            bool inside;
+            ray_init(a, id, p[c], d[c], t0[c], inside);
+            p[c][0] = r.p[0]; p[c][1] = r.p[1]; p[c][2] = r.p[2];
+            t[c] = r.t;
+            steps[c] = r.steps;
+            rid[c] = id;
+            fresh[c] = true;
+          } else {
+            rid[c] = -1;
+          }
+        } else {
+          while (nid < total) {
+            bool inside;
+            ray_init(a, nid, p[c], d[c], t0[c], inside);
+            if (inside) break;
+            // the ray never meets the bounding sphere: a miss, then the next ray
+            a.depth[nid] = -1.0f;
+            a.steps[nid] = 0;
+            nid = (int64_t)atomicAdd(a.next_ray, 1ull);
+          }
+          rid[c] = nid < total ? nid : -1;
+          t[c] = 0.f;
+          steps[c] = 0;
         }
-        rid[c] = nid < total ? nid : -1;
-        t[c] = 0.f;
-        steps[c] = 0;
       }
     }
     // group-1 lanes copy the slot state of their group-0 partner so that both
@@ -109,7 +137,7 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
       x[c][2] = act ? p[c][2] : 0.f;
     }
     // one call site for every level keeps a single copy of each width's code
-    for (int l = 0; l < a.n_levels; ++l) {
+    for (int l = 0; l < n_eval; ++l) {
       // column blocks with a ray that needs level l
       unsigned cols = ~0u;
       if (l > 0) {
@@ -128,7 +156,8 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
           F[c] = r[c];
           Fsum[c] = r[c];
           need[c] = rid[c] >= 0 && a.n_levels > 1 && r[c] < a.band_scale * a.deltas[0];
-          if (lane < S::ROWS && rid[c] >= 0) evals[0]++;
+          if (lane < S::ROWS && rid[c] >= 0 && !fresh[c]) evals[0]++;
+          fresh[c] = false;
         }
       } else {
 #pragma unroll
@@ -139,6 +168,28 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
           }
           need[c] = need[c] && l + 1 < a.n_levels && fabsf(Fsum[c]) < a.band_scale * a.deltas[l];
         }
+      }
+    }
+    // coarse phase: rays whose coarse value is in the coarse band are parked
+    // (before their step) for the band phase
+    if (a.phase == TRACE_COARSE) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const bool park = rid[c] >= 0 && F[c] < a.band_scale * a.deltas[0];
+        const uint64_t m = __ballot(lane < S::ROWS && park);
+        if (m == 0) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.n_parked, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (lane < S::ROWS && park) {
+          const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
+          ParkedRay& r = a.park[base + __popcll(below)];
+          r.rid = (uint32_t)rid[c];
+          r.steps = steps[c];
+          r.t = t[c];
+          r.p[0] = p[c][0]; r.p[1] = p[c][1]; r.p[2] = p[c][2];
+        }
+        if (park) rid[c] = -1;
       }
     }
     // step, hit and exit tests
@@ -421,23 +472,39 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   // regions take 128 KB of the CU's LDS)
   const int blocks = cus;
   const size_t lds = 4 * TRACE_HAND_BYTES;
-  switch (st->shape) {
-    case SH_32x32x16_BF16:
-      rc = nc == 2 ? launch_trace<S32x32x16_bf16, 2>(blocks, lds, s, a)
-                   : launch_trace<S32x32x16_bf16, 1>(blocks, lds, s, a);
-      break;
-    case SH_32x32x16_F16:
-      rc = nc == 2 ? launch_trace<S32x32x16_f16, 2>(blocks, lds, s, a)
-                   : launch_trace<S32x32x16_f16, 1>(blocks, lds, s, a);
-      break;
-    case SH_16x16x4_F32:
-      rc = launch_trace<S16x16x4_f32, 1>(blocks, lds, s, a);
-      break;
-    default:
-      set_error("mp_trace: unsupported fragment shape %d", st->shape);
-      return MP_EUNSUPPORTED;
+  auto launch = [&](const TraceArgs& ta) -> int {
+    switch (st->shape) {
+      case SH_32x32x16_BF16:
+        return nc == 2 ? launch_trace<S32x32x16_bf16, 2>(blocks, lds, s, ta)
+                       : launch_trace<S32x32x16_bf16, 1>(blocks, lds, s, ta);
+      case SH_32x32x16_F16:
+        return nc == 2 ? launch_trace<S32x32x16_f16, 2>(blocks, lds, s, ta)
+                       : launch_trace<S32x32x16_f16, 1>(blocks, lds, s, ta);
+      case SH_16x16x4_F32:
+        return launch_trace<S16x16x4_f32, 1>(blocks, lds, s, ta);
+      default:
+        set_error("mp_trace: unsupported fragment shape %d", st->shape);
+        return MP_EUNSUPPORTED;
+    }
+  };
+  if (st->n_levels == 1) {
+    rc = launch(a);
+    if (rc != MP_OK) return rc;
+  } else {
+    // coarse phase, then the band phase over the rays it parked; the parked
+    // records (24 B per ray) take the compacted-hit and gradient regions,
+    // which are written only after tracing
+    static_assert(sizeof(ParkedRay) == 24, "parked rays must fit the 24 B per ray of cpos + grad");
+    a.park = reinterpret_cast<ParkedRay*>(cpos);
+    a.n_parked = &ctr[9];
+    a.phase = TRACE_COARSE;
+    rc = launch(a);
+    if (rc != MP_OK) return rc;
+    a.phase = TRACE_BAND;
+    a.next_ray = &ctr[10];
+    rc = launch(a);
+    if (rc != MP_OK) return rc;
   }
-  if (rc != MP_OK) return rc;
   hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, depth,
                      hit_pos, total, ids, cpos, &ctr[8]);
   MP_LAUNCHED("k_collect_hits");
