@@ -5,22 +5,14 @@
 
 namespace mp {
 
-// State of a ray parked by the coarse phase where it entered the coarse band
-// (direction and entry distance are regenerated from the id).
+// State of a ray parked by one launch for the next (direction and entry
+// distance are regenerated from the id).
 struct ParkedRay {
   uint32_t rid;
   int32_t steps;
   float t;
   float p[3];
 };
-
-// Ray grouping of one k_trace launch.  TRACE_ALL: every ray with every level
-// (single pass).  TRACE_COARSE: every ray with the coarse level only; a ray is
-// parked, before its step, once its coarse value enters the coarse band.
-// TRACE_BAND: the parked rays, with every level.  Coarse then band gives each
-// ray the same evaluations and steps as the single pass; it only keeps the
-// rays that need residual levels together in the tiles.
-enum TracePhase { TRACE_ALL = 0, TRACE_COARSE = 1, TRACE_BAND = 2 };
 
 struct TraceArgs {
   const float* cams;  // [n_cams][32]: inverse view then inverse projection, column-major 4x4
@@ -40,9 +32,17 @@ struct TraceArgs {
   float* hit_pos;                // [rays][3] (scratch)
   uint32_t* rgba;                // [rays] RGBA8
   float* normals;                // [rays][3] or NULL
-  int phase;                     // TracePhase
-  ParkedRay* park;               // [rays] parked rays (TRACE_COARSE writes, TRACE_BAND reads)
-  unsigned long long* n_parked;  // parked-ray count
+  // Level-by-level launches (eval_levels > 0): the launch evaluates levels
+  // 0 .. eval_levels - 1 and parks, before its step, every ray that needs
+  // level eval_levels; the next launch traces the parked rays.  Each ray gets
+  // the same evaluations and steps as in the single pass (eval_levels 0: every
+  // level, the ray queue, no parking); only the rays that need the wider
+  // levels stay together in the tiles.
+  int eval_levels;
+  const ParkedRay* park_in;               // rays of this launch (NULL: the ray queue)
+  const unsigned long long* n_park_in;    // their count
+  ParkedRay* park_out;                    // [rays] rays parked for the next launch, or NULL
+  unsigned long long* n_park_out;         // their count
 };
 
 }  // namespace mp

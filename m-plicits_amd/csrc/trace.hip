@@ -9,10 +9,10 @@
 // ray counter, so the matrix cores only ever see live rays.  The coarse level
 // is evaluated for every live ray; a residual level only when some ray of the
 // tile is inside the band of the partial sum before it.  A stack with residual
-// levels is traced in two launches: every ray on the coarse level until it
-// enters the coarse band, then the rays parked there with every level, so that
-// far-field rays (coarse only) and near-surface rays (every level) do not
-// share tiles.
+// levels is traced level by level: every ray on the coarse level until it
+// enters the coarse band, then the rays parked there on the levels up to the
+// next one until they enter that level's band, and so on, so that far-field
+// rays (coarse only) and near-surface rays (every level) do not share tiles.
 //
 // Step rule (the renderer's, with a hit test):
 //   F_0 = f_0(p).  Outside the coarse band (F_0 >= s * delta_0, s = band_scale,
@@ -51,14 +51,14 @@ template <class S, int NC>
 __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag* hand) {
   const int lane = threadIdx.x & 63;
   const int col = lane % S::ROWS;
-  // the band phase's queue is the list the coarse phase parked
-  const int64_t total = a.phase == TRACE_BAND ? (int64_t)*a.n_parked : a.total_rays;
-  const int n_eval = a.phase == TRACE_COARSE ? 1 : a.n_levels;
+  // a launch after the first traces the rays the launch before parked
+  const int64_t total = a.park_in ? (int64_t)*a.n_park_in : a.total_rays;
+  const int n_eval = a.eval_levels > 0 ? a.eval_levels : a.n_levels;
   int64_t rid[NC];
   float p[NC][3], d[NC][3], t[NC], t0[NC];
   int steps[NC];
-  // a parked ray's first coarse evaluation repeats the one it was parked at:
-  // not counted again
+  // a parked ray's first evaluation repeats, at the same point, the levels the
+  // launch before evaluated (0 .. n_eval - 2): those are not counted again
   bool fresh[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -81,9 +81,9 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
       if (lane < S::ROWS && empty) {
         const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
         int64_t nid = (int64_t)(base + __popcll(below));
-        if (a.phase == TRACE_BAND) {
+        if (a.park_in) {
           if (nid < total) {
-            const ParkedRay& r = a.park[nid];
+            const ParkedRay& r = a.park_in[nid];
             const int64_t id = r.rid;
             bool inside;
             ray_init(a, id, p[c], d[c], t0[c], inside);
@@ -157,33 +157,34 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
           Fsum[c] = r[c];
           need[c] = rid[c] >= 0 && a.n_levels > 1 && r[c] < a.band_scale * a.deltas[0];
           if (lane < S::ROWS && rid[c] >= 0 && !fresh[c]) evals[0]++;
-          fresh[c] = false;
         }
       } else {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           if (need[c]) {
             Fsum[c] += r[c];
-            if (lane < S::ROWS) evals[l]++;
+            if (lane < S::ROWS && !(fresh[c] && l < n_eval - 1)) evals[l]++;
           }
           need[c] = need[c] && l + 1 < a.n_levels && fabsf(Fsum[c]) < a.band_scale * a.deltas[l];
         }
       }
     }
-    // coarse phase: rays whose coarse value is in the coarse band are parked
-    // (before their step) for the band phase
-    if (a.phase == TRACE_COARSE) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) fresh[c] = false;
+    // rays that need the next launch's level are parked before their step
+    // (need[c]: level n_eval is due at this point)
+    if (a.park_out) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        const bool park = rid[c] >= 0 && F[c] < a.band_scale * a.deltas[0];
+        const bool park = need[c];
         const uint64_t m = __ballot(lane < S::ROWS && park);
         if (m == 0) continue;
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.n_parked, (unsigned long long)__popcll(m));
+        if (lane == 0) base = atomicAdd(a.n_park_out, (unsigned long long)__popcll(m));
         base = __shfl(base, 0);
         if (lane < S::ROWS && park) {
           const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
-          ParkedRay& r = a.park[base + __popcll(below)];
+          ParkedRay& r = a.park_out[base + __popcll(below)];
           r.rid = (uint32_t)rid[c];
           r.steps = steps[c];
           r.t = t[c];
@@ -429,7 +430,9 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   const size_t off_ids = off_pos + (size_t)total * 12;
   const size_t off_cpos = (off_ids + (size_t)total * 4 + 255) & ~(size_t)255;
   const size_t off_grad = off_cpos + (size_t)total * 12;
-  int rc = ensure_scratch(st, off_grad + (size_t)total * 12);
+  // | second parked-ray list [total] (stacks of three or more levels)
+  const size_t off_park2 = (off_grad + (size_t)total * 12 + 255) & ~(size_t)255;
+  int rc = ensure_scratch(st, st->n_levels >= 3 ? off_park2 + (size_t)total * 24 : off_grad + (size_t)total * 12);
   if (rc != MP_OK) return rc;
   char* base = reinterpret_cast<char*>(st->scratch);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base);
@@ -437,6 +440,7 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   uint32_t* ids = reinterpret_cast<uint32_t*>(base + off_ids);
   float* cpos = reinterpret_cast<float*>(base + off_cpos);
   float* grad = reinterpret_cast<float*>(base + off_grad);
+  char* park2 = st->n_levels >= 3 ? base + off_park2 : nullptr;
 
   TraceArgs a{};
   a.cams = cams;
@@ -491,19 +495,22 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
     rc = launch(a);
     if (rc != MP_OK) return rc;
   } else {
-    // coarse phase, then the band phase over the rays it parked; the parked
-    // records (24 B per ray) take the compacted-hit and gradient regions,
-    // which are written only after tracing
+    // one launch per level: launch k evaluates levels 0 .. k - 1 and parks the
+    // rays that need level k for launch k + 1.  Two parked lists alternate
+    // (24 B per ray each; a ray is parked at most once per launch): the first
+    // takes the compacted-hit and gradient regions, written only after tracing.
     static_assert(sizeof(ParkedRay) == 24, "parked rays must fit the 24 B per ray of cpos + grad");
-    a.park = reinterpret_cast<ParkedRay*>(cpos);
-    a.n_parked = &ctr[9];
-    a.phase = TRACE_COARSE;
-    rc = launch(a);
-    if (rc != MP_OK) return rc;
-    a.phase = TRACE_BAND;
-    a.next_ray = &ctr[10];
-    rc = launch(a);
-    if (rc != MP_OK) return rc;
+    ParkedRay* lists[2] = {reinterpret_cast<ParkedRay*>(cpos), reinterpret_cast<ParkedRay*>(park2)};
+    for (int k = 1; k <= st->n_levels; ++k) {
+      a.eval_levels = k;
+      a.park_in = k > 1 ? lists[k % 2] : nullptr;            // written by launch k - 1
+      a.n_park_in = k > 1 ? &ctr[8 + k - 1] : nullptr;
+      a.park_out = k < st->n_levels ? lists[(k + 1) % 2] : nullptr;
+      a.n_park_out = k < st->n_levels ? &ctr[8 + k] : nullptr;
+      a.next_ray = k > 1 ? &ctr[16 + k] : &ctr[0];
+      rc = launch(a);
+      if (rc != MP_OK) return rc;
+    }
   }
   hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, depth,
                      hit_pos, total, ids, cpos, &ctr[8]);
