@@ -174,8 +174,19 @@ struct LevelCore {
   // the derivative cos(r_0) 2 pi r_t (every lane must call it: it shuffles).
   static __device__ __forceinline__ float act_rev(float v, int lane) {
     if constexpr (TANGENT) {
-      const float r0 = __shfl(v, lane & ~3);
-      return tangent_of(lane) == 0 ? sin_rev(v) : cos_rev(r0) * (6.28318530717958648f * v);
+      // value column of the quad (lanes 4p .. 4p + 3): DPP quad broadcast of its first lane
+      const float r0 = __builtin_bit_cast(
+          float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x00, 0xF, 0xF, false));
+      const bool value = tangent_of(lane) == 0;
+      if constexpr (MFMA_FIRST) {
+        // 16-bit shapes: one transcendental per lane, cos(2 pi r) = sin(2 pi (r + 1/4)) (the
+        // rounding of r + 1/4, half an ulp of r, stays far below the rounding of the 16-bit
+        // operands every tangent has passed through)
+        const float s = sin_rev(value ? v : r0 + 0.25f);
+        return value ? s : s * (6.28318530717958648f * v);
+      } else {
+        return value ? sin_rev(v) : cos_rev(r0) * (6.28318530717958648f * v);
+      }
     } else {
       return sin_rev(v);
     }
