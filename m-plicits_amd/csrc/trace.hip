@@ -69,20 +69,26 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
   unsigned long long evals[MP_MAX_LEVELS] = {0, 0, 0, 0};
 
   while (true) {
-    // refill empty slots from the global queue (group-0 lanes vote)
+    // refill empty slots from the global queue (group-0 lanes vote), in rounds
+    // of one reservation per wave: a ray that misses the bounding sphere is
+    // written out at once and its slot votes again in the next round (every
+    // round takes at least one ray id, so the rounds end)
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const bool empty = rid[c] < 0;
-      const uint64_t m = __ballot(lane < S::ROWS && empty);
-      if (m == 0) continue;
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(a.next_ray, (unsigned long long)__popcll(m));
-      base = __shfl(base, 0);
-      if (lane < S::ROWS && empty) {
-        const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
-        int64_t nid = (int64_t)(base + __popcll(below));
-        if (a.park_in) {
-          if (nid < total) {
+      bool want = rid[c] < 0;
+      while (true) {
+        const uint64_t m = __ballot(lane < S::ROWS && want);
+        if (m == 0) break;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.next_ray, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (lane < S::ROWS && want) {
+          const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
+          const int64_t nid = (int64_t)(base + __popcll(below));
+          want = false;
+          if (nid >= total) {
+            rid[c] = -1;  // the queue is empty
+          } else if (a.park_in) {
             const ParkedRay& r = a.park_in[nid];
             const int64_t id = r.rid;
             bool inside;
@@ -93,21 +99,19 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
             rid[c] = id;
             fresh[c] = true;
           } else {
-            rid[c] = -1;
-          }
-        } else {
-          while (nid < total) {
             bool inside;
             ray_init(a, nid, p[c], d[c], t0[c], inside);
-            if (inside) break;
-            // the ray never meets the bounding sphere: a miss, then the next ray
-            a.depth[nid] = -1.0f;
-            a.steps[nid] = 0;
-            nid = (int64_t)atomicAdd(a.next_ray, 1ull);
+            if (inside) {
+              rid[c] = nid;
+              t[c] = 0.f;
+              steps[c] = 0;
+            } else {
+              // the ray never meets the bounding sphere: a miss
+              a.depth[nid] = -1.0f;
+              a.steps[nid] = 0;
+              want = true;
+            }
           }
-          rid[c] = nid < total ? nid : -1;
-          t[c] = 0.f;
-          steps[c] = 0;
         }
       }
     }
