@@ -102,3 +102,33 @@ def test_render_cli_benchmark_line(tmp_path, capsys, flags, iters):
     assert f"BENCHMARK resolution=64x48 iters={iters} delta=0.0500" in out and "frames=2 avg_fps=" in out
     assert open(tmp_path / "img.ppm", "rb").read(11) == b"P6 64 48 25"
     assert open(tmp_path / "benchmark.csv").read().startswith(f"64x48,{iters},0.0500,")
+
+
+@pytest.mark.parametrize("n_levels", [1, 3])
+def test_rays_that_miss_the_bounding_sphere(n_levels):
+    """A camera looking away from the scene: every ray misses the bounding
+    sphere (depth -1, no steps, no evaluations; with residual levels the
+    level-by-level launches after the first find nothing parked).  Beside a
+    camera that sees the surface, that camera's rays equal its own frame."""
+    from mplicits.synthetic import look_at, perspective
+    coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
+    levels = [coarse]
+    if n_levels == 3:
+        levels += [seeded_siren([128] * 3, 60.0, seed=42, last_scale=0.1).cuda(),
+                   seeded_siren([64] * 2, 90.0, seed=43, last_scale=0.1).cuda()]
+    deltas = [0.05, 0.02][:n_levels - 1]
+    st = LevelStack(levels, "bf16")
+    away = (np.linalg.inv(look_at((0.0, 0.0, 3.0), target=(0.0, 0.0, 10.0))),
+            np.linalg.inv(perspective(45.0, 1.5)))
+    W, H = 48, 32
+    out = trace(st, [away], W, H, deltas=deltas)
+    assert bool((out["depth"] == -1).all()) and bool((out["steps"] == 0).all())
+    assert out["hits"] == 0 and out["evals"] == [0] * n_levels
+    see = orbit_cameras(1, seed=3)[0]
+    alone = trace(st, [see], W, H, deltas=deltas)
+    mixed = trace(st, [away, see], W, H, deltas=deltas)
+    assert alone["hits"] > 100
+    for k in ("depth", "steps", "rgba", "normals"):
+        assert torch.equal(mixed[k][1], alone[k][0]), k
+        assert torch.equal(mixed[k][0], out[k][0]), k
+    assert mixed["hits"] == alone["hits"] and mixed["evals"] == alone["evals"]
