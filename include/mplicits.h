@@ -161,7 +161,9 @@ int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, v
  * miss), steps, rgba (RGBA8: Lambert shading of the analytic normal, white
  * background), normals [rays][3] (unit, zero for misses; may be NULL).
  * stats (host, MP_MAX_LEVELS + 1 entries, may be NULL): SDF evaluations per
- * level, then the number of hits.  Host-sync point.  Allocates device scratch. */
+ * level, then the number of hits.  Host-sync point.  Allocates device scratch
+ * owned by the stack and kept for later calls: 40 B per ray, 64 B per ray for
+ * stacks of three or more levels (the level-by-level launches park rays). */
 typedef struct mp_trace_params {
   int width, height;
   int max_steps;       /* e.g. 128 */
