@@ -224,8 +224,13 @@ def bench_trace(levels, precisions, reps=2, check=False):
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / reps
             key = prec if name == "coarse" else f"{name}_{prec}"
+            # algorithmic FLOP of the counted step evaluations (normals and
+            # shading excluded) over the whole frame-set time
+            flop = sum(e * level_flops(m.hidden_widths()) for e, m in zip(r["evals"], lv))
             out[key] = {"rays_per_s": round(16 * 1920 * 1080 / dt, 1), "ms": round(dt * 1e3, 2),
-                        "hits": r["hits"], "evals": r["evals"]}
+                        "hits": r["hits"], "evals": r["evals"],
+                        "eval_tflops": round(flop / dt / 1e12, 1),
+                        "eval_frac_of_peak": round(flop / dt / 1e12 / PEAK_TFLOPS[prec], 4)}
             if check:
                 out[key]["vs_fp64"] = trace_errors(lv, cam_list, 1920, 1080, r)
             del r
