@@ -132,3 +132,30 @@ def test_rays_that_miss_the_bounding_sphere(n_levels):
         assert torch.equal(mixed[k][1], alone[k][0]), k
         assert torch.equal(mixed[k][0], out[k][0]), k
     assert mixed["hits"] == alone["hits"] and mixed["evals"] == alone["evals"]
+
+
+def test_three_level_stack_matches_oracle_step_rule():
+    """Three levels (the tracer's level-by-level launches, a 512-wide fine
+    level) against the FP64 restatement of the step rule."""
+    coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
+    levels = [coarse, seeded_siren([128] * 3, 60.0, seed=42, last_scale=0.1).cuda(),
+              seeded_siren([512] * 3, 90.0, seed=44, last_scale=0.1).cuda()]
+    deltas = [0.05, 0.02]
+    cams = orbit_cameras(2, seed=11)
+    W, H = 64, 48
+    out = trace(LevelStack(levels, "fp32"), cams, W, H, deltas=deltas)
+    depth = out["depth"].cpu().numpy().reshape(-1)
+    rng = np.random.default_rng(1)
+    ids = rng.choice(len(depth), 1000, replace=False)  # the surface covers ~8% of the frame
+    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
+    layers = [osiren.layers_from_model(m) for m in levels]
+    hit, t, _ = otrace.trace(layers, deltas, P[inside], D[inside])
+    ref_depth = np.full(len(ids), -1.0)
+    ref_depth[np.nonzero(inside)[0][hit]] = (t0[inside] + t)[hit]
+    got = depth[ids]
+    agree = (got >= 0) == (ref_depth >= 0)
+    assert agree.mean() > 0.97, agree.mean()
+    both = (got >= 0) & (ref_depth >= 0)
+    assert both.sum() > 50
+    assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
+    assert all(e > 0 for e in out["evals"]) and out["evals"][0] > out["evals"][1] > 0
