@@ -368,6 +368,9 @@ def bench_batch(precision="bf16", reps=3, n_models=24):
     stats = tr.stats()
     tr.close()
     rays = n_models * 1280 * 720
+    # algorithmic FLOP of the counted step evaluations (normals excluded)
+    tr_flop = sum(e * level_flops(lv.hidden_widths()) for m, s in zip(work, stats)
+                  for e, lv in zip(s["evals"], m["levels"]))
     widths = {}
     for m in work:
         widths[m["width"]] = widths.get(m["width"], 0) + 1
@@ -379,6 +382,8 @@ def bench_batch(precision="bf16", reps=3, n_models=24):
             "mc_ms": round(mc_ms, 2), "faces": faces,
             "trace_ms": round(tr_ms, 2), "rays": rays, "rays_per_s": round(rays / (tr_ms * 1e-3), 1),
             "hits": int(sum(s["hits"] for s in stats)),
+            "trace_evals_per_level": [int(sum(s["evals"][l] for s in stats if l < len(s["evals"]))) for l in range(3)],
+            "trace_eval_tflops": round(tr_flop / (tr_ms * 1e-3) / 1e12, 1),
             "graphs": "extraction and tracing each captured once, timed over replays"}
 
 
