@@ -159,3 +159,37 @@ def test_three_level_stack_matches_oracle_step_rule():
     assert both.sum() > 50
     assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
     assert all(e > 0 for e in out["evals"]) and out["evals"][0] > out["evals"][1] > 0
+
+
+def test_four_level_stack_matches_oracle_step_rule():
+    """Four levels (the most a stack holds): the third launch parks into the
+    list the second one read, the fourth traces it."""
+    coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
+    levels = [coarse, seeded_siren([128] * 3, 60.0, seed=45, last_scale=0.1),
+              seeded_siren([64] * 2, 90.0, seed=46, last_scale=0.1),
+              seeded_siren([128] * 2, 120.0, seed=47, last_scale=0.05)]
+    # residuals without output bias: stacked constant offsets would move the
+    # zero of the sum out of the nested bands, and no ray would hit
+    with torch.no_grad():
+        for m in levels[1:]:
+            m.net[-1][0].bias.zero_()
+    levels = [levels[0]] + [m.cuda() for m in levels[1:]]
+    deltas = [0.05, 0.03, 0.02]
+    cams = orbit_cameras(2, seed=13)
+    W, H = 64, 48
+    out = trace(LevelStack(levels, "fp32"), cams, W, H, deltas=deltas)
+    depth = out["depth"].cpu().numpy().reshape(-1)
+    rng = np.random.default_rng(2)
+    ids = rng.choice(len(depth), 1000, replace=False)  # the surface covers ~8% of the frame
+    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
+    layers = [osiren.layers_from_model(m) for m in levels]
+    hit, t, _ = otrace.trace(layers, deltas, P[inside], D[inside])
+    ref_depth = np.full(len(ids), -1.0)
+    ref_depth[np.nonzero(inside)[0][hit]] = (t0[inside] + t)[hit]
+    got = depth[ids]
+    agree = (got >= 0) == (ref_depth >= 0)
+    assert agree.mean() > 0.97, agree.mean()
+    both = (got >= 0) & (ref_depth >= 0)
+    assert both.sum() > 50
+    assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
+    assert len(out["evals"]) == 4 and all(e > 0 for e in out["evals"])
