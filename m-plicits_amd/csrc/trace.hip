@@ -271,24 +271,48 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const TraceArgs* __restrict_
   }
 }
 
-// Compact the hit rays: ids and positions.
-__global__ void k_collect_hits(const float* depth, const float* hit_pos, int64_t n, uint32_t* ids,
-                               float* pos, unsigned long long* count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const bool hit = i < n && depth[i] >= 0.f;
-  const uint64_t m = __ballot(hit);
-  if (!m) return;
-  unsigned long long base = 0;
-  if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(m));
-  base = __shfl(base, 0);
-  if (hit) {
-    const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    const unsigned long long k = base + __popcll(below);
-    ids[k] = (uint32_t)i;
-    pos[3 * k + 0] = hit_pos[3 * i + 0];
-    pos[3 * k + 1] = hit_pos[3 * i + 1];
-    pos[3 * k + 2] = hit_pos[3 * i + 2];
+// Compact the hit rays: ids and positions.  Each workgroup takes
+// COLLECT_PER_BLOCK consecutive rays (COLLECT_K per thread, strided by the
+// block size) and reserves its output range with one atomic: a reservation per
+// wave on the one counter serialises at the L2.
+static constexpr int COLLECT_K = 8;
+static constexpr int COLLECT_PER_BLOCK = 256 * COLLECT_K;
+
+__global__ __launch_bounds__(256) void k_collect_hits(const float* depth, const float* hit_pos, int64_t n,
+                                                      uint32_t* ids, float* pos, unsigned long long* count) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t b0 = (int64_t)blockIdx.x * COLLECT_PER_BLOCK;
+  uint64_t m[COLLECT_K];
+  int wave_hits = 0;
+#pragma unroll
+  for (int k = 0; k < COLLECT_K; ++k) {
+    const int64_t i = b0 + k * 256 + threadIdx.x;
+    m[k] = __ballot(i < n && depth[i] >= 0.f);
+    wave_hits += __popcll(m[k]);
+  }
+  __shared__ int wsum[4];
+  __shared__ unsigned long long base;
+  if (lane == 0) wsum[wave] = wave_hits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    base = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long k0 = base;
+  for (int w = 0; w < wave; ++w) k0 += wsum[w];
+  const uint64_t below_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int k = 0; k < COLLECT_K; ++k) {
+    if ((m[k] >> lane) & 1ull) {
+      const int64_t i = b0 + k * 256 + threadIdx.x;
+      const unsigned long long o = k0 + __popcll(m[k] & below_mask);
+      ids[o] = (uint32_t)i;
+      pos[3 * o + 0] = hit_pos[3 * i + 0];
+      pos[3 * o + 1] = hit_pos[3 * i + 1];
+      pos[3 * o + 2] = hit_pos[3 * i + 2];
+    }
+    k0 += __popcll(m[k]);
   }
 }
 
@@ -516,7 +540,8 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
       if (rc != MP_OK) return rc;
     }
   }
-  hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, depth,
+  hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + COLLECT_PER_BLOCK - 1) / COLLECT_PER_BLOCK)),
+                     dim3(256), 0, s, depth,
                      hit_pos, total, ids, cpos, &ctr[8]);
   MP_LAUNCHED("k_collect_hits");
   hipLaunchKernelGGL(k_background, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, rgba,
