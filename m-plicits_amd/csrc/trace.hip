@@ -67,24 +67,35 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
   }
   const float R2 = a.bound_radius * a.bound_radius * 1.0001f;
   unsigned long long evals[MP_MAX_LEVELS] = {0, 0, 0, 0};
+  // ray ids this wave has reserved from the global queue and not handed out
+  // yet: [pool, pool_end) (wave-uniform)
+  constexpr unsigned long long POOL = 64;
+  unsigned long long pool = 0, pool_end = 0;
 
   while (true) {
-    // refill empty slots from the global queue (group-0 lanes vote), in rounds
-    // of one reservation per wave: a ray that misses the bounding sphere is
-    // written out at once and its slot votes again in the next round (every
-    // round takes at least one ray id, so the rounds end)
+    // refill empty slots (group-0 lanes vote) from the wave's pool, reserving
+    // POOL more ids from the global queue when it runs dry, in rounds: a ray
+    // that misses the bounding sphere is written out at once and its slot
+    // votes again in the next round (every round hands out at least one id, so
+    // the rounds end)
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       bool want = rid[c] < 0;
       while (true) {
         const uint64_t m = __ballot(lane < S::ROWS && want);
         if (m == 0) break;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.next_ray, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (lane < S::ROWS && want) {
-          const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
-          const int64_t nid = (int64_t)(base + __popcll(below));
+        if (pool == pool_end) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(a.next_ray, POOL);
+          pool = __shfl(base, 0);
+          pool_end = pool + POOL;
+        }
+        const unsigned long long take =
+            (unsigned long long)__popcll(m) < pool_end - pool ? (unsigned long long)__popcll(m) : pool_end - pool;
+        const uint64_t below = (col == 0) ? 0ull : (m & ((~0ull) >> (64 - col)));
+        const unsigned long long rank = (unsigned long long)__popcll(below);
+        if (lane < S::ROWS && want && rank < take) {
+          const int64_t nid = (int64_t)(pool + rank);
           want = false;
           if (nid >= total) {
             rid[c] = -1;  // the queue is empty
@@ -113,6 +124,7 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
             }
           }
         }
+        pool += take;
       }
     }
     // group-1 lanes copy the slot state of their group-0 partner so that both
