@@ -5,8 +5,9 @@
 // SIREN GEMM chain of each level on all pixels every step) and
 // IterateSphereTracing_kernel (renderer/cuda/kernels.cu:315-398).  Each wave
 // holds a tile of rays (one per MFMA column); after every step the rays that
-// hit or left the scene are written out and their slots refilled from a global
-// ray counter, so the matrix cores only ever see live rays.  The coarse level
+// hit or left the scene are written out and their slots refilled from the
+// wave's pool of ray ids (reserved from a global ray counter 64 at a time), so
+// the matrix cores only ever see live rays.  The coarse level
 // is evaluated for every live ray; a residual level only when some ray of the
 // tile is inside the band of the partial sum before it.  A stack with residual
 // levels is traced level by level: every ray on the coarse level until it
