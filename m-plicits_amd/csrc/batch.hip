@@ -204,12 +204,8 @@ template <class S>
 static int run_level_stage(mp_batch* p, int stage, hipStream_t s) {
   constexpr size_t lds = jobs_lds_bytes<S>();
   static_assert(lds <= 160 * 1024, "LDS budget of one workgroup");
-  static bool attr_set = false;
-  if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level_jobs<S>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> attr_devs{0};
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_level_jobs<S>), (int)lds));
   hipLaunchKernelGGL(k_level_jobs<S>, dim3(p->blocks), dim3(256), lds, s,
                      reinterpret_cast<const LevelJob*>(p->level_tab[stage]), p->n_level_jobs[stage],
                      &p->counters[stage]);

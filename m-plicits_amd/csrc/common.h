@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
+
+#include <atomic>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -16,6 +18,19 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 const char* get_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize) once per kernel and device:
+// `done` is the kernel's set of device ids (bit per id < 64) already configured.
+inline hipError_t set_lds_once(std::atomic<uint64_t>& done, const void* fn, int bytes) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const uint64_t bit = dev < 64 ? (1ull << dev) : 0ull;
+  if (bit && (done.load(std::memory_order_relaxed) & bit)) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess) done.fetch_or(bit);
+  return e;
+}
 
 }  // namespace mp
 

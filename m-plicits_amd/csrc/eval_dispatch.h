@@ -43,5 +43,15 @@ int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t
 int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
                 hipStream_t stream);
 
+// The band-culled extraction pipeline behind mp_extract_volume.  scratch ==
+// NULL: work lists in the stack's own (grown on demand) scratch; otherwise the
+// caller's device buffer of scratch_cap bytes (a call needing more is
+// rejected before any launch).  `who` names the entry point in messages.
+int extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const float* deltas, int n_deltas,
+                   float* vol, int64_t* level_counts, void* stream, void* scratch, size_t scratch_cap,
+                   const char* who);
+// Scratch bytes extract_volume needs for a culled extraction of n grid points.
+size_t extract_scratch_bytes(int n_levels, int64_t n, int64_t chunk);
+
 
 }  // namespace mp

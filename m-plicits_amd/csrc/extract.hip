@@ -144,15 +144,33 @@ int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, v
 
 int mp_extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const float* deltas,
                       int n_deltas, float* vol, int64_t* level_counts, void* stream) {
-  MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_extract_volume: stack is NULL");
-  MP_REQUIRE(N >= 2 && N <= 1625, MP_EINVAL, "mp_extract_volume: N must be in [2, 1625], got %d", N);
+  return mp::extract_volume(st, N, grid_mode, chunk, deltas, n_deltas, vol, level_counts, stream, nullptr, 0,
+                            "mp_extract_volume");
+}
+
+}  // extern "C"
+
+namespace mp {
+
+size_t extract_scratch_bytes(int n_levels, int64_t n, int64_t chunk) {
+  const int64_t nchunks = (n + chunk - 1) / chunk;
+  const size_t off_any = 256;
+  const size_t off_cand = (off_any + (size_t)(n_levels - 1) * nchunks + 255) & ~(size_t)255;
+  return off_cand + (size_t)n * 8;
+}
+
+int extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const float* deltas, int n_deltas,
+                   float* vol, int64_t* level_counts, void* stream, void* scratch, size_t scratch_cap,
+                   const char* who) {
+  MP_REQUIRE(st != nullptr, MP_EINVAL, "%s: stack is NULL", who);
+  MP_REQUIRE(N >= 2 && N <= 1625, MP_EINVAL, "%s: N must be in [2, 1625], got %d", who, N);
   MP_REQUIRE(grid_mode == MP_GRID_REFERENCE || grid_mode == MP_GRID_LATTICE, MP_EINVAL,
-             "mp_extract_volume: unknown grid mode %d", grid_mode);
-  MP_REQUIRE(chunk >= 1, MP_EINVAL, "mp_extract_volume: chunk must be >= 1");
-  MP_REQUIRE(vol != nullptr, MP_EINVAL, "mp_extract_volume: vol is NULL");
+             "%s: unknown grid mode %d", who, grid_mode);
+  MP_REQUIRE(chunk >= 1, MP_EINVAL, "%s: chunk must be >= 1", who);
+  MP_REQUIRE(vol != nullptr, MP_EINVAL, "%s: vol is NULL", who);
   MP_REQUIRE(n_deltas == 0 || n_deltas == st->n_levels - 1, MP_EINVAL,
-             "mp_extract_volume: need 0 or %d deltas, got %d", st->n_levels - 1, n_deltas);
-  MP_REQUIRE(n_deltas == 0 || deltas != nullptr, MP_EINVAL, "mp_extract_volume: deltas is NULL");
+             "%s: need 0 or %d deltas, got %d", who, st->n_levels - 1, n_deltas);
+  MP_REQUIRE(n_deltas == 0 || deltas != nullptr, MP_EINVAL, "%s: deltas is NULL", who);
   hipStream_t s = as_stream(stream);
   const int64_t n = (int64_t)N * N * N;
   const float h = voxel_size(N);
@@ -186,9 +204,18 @@ int mp_extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const f
   const size_t off_any = 256;
   const size_t off_cand = (off_any + (size_t)(L - 1) * nchunks + 255) & ~(size_t)255;
   const size_t off_list = off_cand + (size_t)n * 4;
-  int rc = ensure_scratch(st, off_list + (size_t)n * 4);
-  if (rc != MP_OK) return rc;
-  char* base = reinterpret_cast<char*>(st->scratch);
+  const size_t need = extract_scratch_bytes(L, n, chunk);  // = off_list + n * 4
+  char* base;
+  if (scratch == nullptr) {
+    int rc = ensure_scratch(st, need);
+    if (rc != MP_OK) return rc;
+    base = reinterpret_cast<char*>(st->scratch);
+  } else {
+    MP_REQUIRE(need <= scratch_cap, MP_EINVAL, "%s: needs %zu bytes of scratch, the caller's buffer holds %zu",
+               who, need, scratch_cap);
+    base = reinterpret_cast<char*>(scratch);
+  }
+  int rc;
   uint32_t* counters = reinterpret_cast<uint32_t*>(base);
   uint8_t* any = reinterpret_cast<uint8_t*>(base + off_any);
   uint32_t* cand = reinterpret_cast<uint32_t*>(base + off_cand);
@@ -246,4 +273,4 @@ int mp_extract_volume(mp_stack* st, int N, int grid_mode, int64_t chunk, const f
   return MP_OK;
 }
 
-}  // extern "C"
+}  // namespace mp

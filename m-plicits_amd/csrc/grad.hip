@@ -224,12 +224,8 @@ static int launch_grad_typed(const LevelDev& L, const float* xyz, int64_t n, flo
   const int64_t per_block = (int64_t)waves * S::ROWS;
   const int64_t blocks = (n + per_block - 1) / per_block;
   const size_t lds = (size_t)waves * MG::LDS_PER_WAVE;
-  static bool attr_set = false;
-  if (!attr_set && lds > 64 * 1024) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level_grad<S, W, G>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> attr_devs{0};
+  if (lds > 64 * 1024) MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_level_grad<S, W, G>), (int)lds));
   hipLaunchKernelGGL((k_level_grad<S, W, G>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, L,
                      xyz, n, f, g, acc, n_dev);
   MP_LAUNCHED("k_level_grad");

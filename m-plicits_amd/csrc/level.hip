@@ -24,12 +24,8 @@ static int launch_typed(const LevelDev& L, const EvalTarget& t, hipStream_t stre
   if (blocks <= 0) return MP_OK;
   MP_REQUIRE(blocks < (1ll << 31), MP_EINVAL, "level launch: %lld points is too many", (long long)t.n);
   static_assert(M::LDS_BYTES <= 160 * 1024, "LDS budget of one workgroup");
-  static bool attr_set = false;
-  if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level<S, W, NC, WAVES>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)M::LDS_BYTES));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> attr_devs{0};
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_level<S, W, NC, WAVES>), (int)M::LDS_BYTES));
   hipLaunchKernelGGL((k_level<S, W, NC, WAVES>), dim3((unsigned)blocks), dim3(64 * WAVES), M::LDS_BYTES, stream, L,
                      t);
   MP_LAUNCHED("k_level");
@@ -127,12 +123,8 @@ static int launch_tangent_typed(const LevelDev& L, const float* xyz, int64_t n, 
   const int64_t blocks = (n + per_block - 1) / per_block;
   if (blocks <= 0) return MP_OK;
   MP_REQUIRE(blocks < (1ll << 31), MP_EINVAL, "gradient launch: %lld points is too many", (long long)n);
-  static bool attr_set = false;
-  if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_level_tangent<S, W, NC>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)M::LDS_BYTES));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> attr_devs{0};
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_level_tangent<S, W, NC>), (int)M::LDS_BYTES));
   hipLaunchKernelGGL((k_level_tangent<S, W, NC>), dim3((unsigned)blocks), dim3(256), M::LDS_BYTES, stream, L, xyz, n,
                      f, g, acc, n_dev);
   MP_LAUNCHED("k_level_tangent");

@@ -226,13 +226,9 @@ __global__ void k_render_shade(const float* __restrict__ cams, int width, int he
 
 template <class S, int NC>
 static int launch_render(const RenderArgs& a, hipStream_t s) {
-  static bool attr_set = false;
+  static std::atomic<uint64_t> attr_devs{0};
   const size_t lds = 4 * TRACE_HAND_BYTES;
-  if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_render<S, NC>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_render<S, NC>), (int)lds));
   const int64_t per_block = 4 * NC * S::ROWS;
   hipLaunchKernelGGL((k_render<S, NC>), dim3((unsigned)((a.total_rays + per_block - 1) / per_block)), dim3(256), lds,
                      s, a);

@@ -419,12 +419,8 @@ __global__ void k_shade_jobs(const TraceArgs* __restrict__ jobs, const TraceOut*
 
 template <class S>
 static int launch_trace_jobs(int blocks, size_t lds, hipStream_t s, const TraceArgs* jobs, int n_jobs) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trace_jobs<S>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> attr_devs{0};
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_trace_jobs<S>), (int)lds));
   hipLaunchKernelGGL((k_trace_jobs<S>), dim3(blocks), dim3(256), lds, s, jobs, n_jobs);
   MP_LAUNCHED("k_trace_jobs");
   return MP_OK;
@@ -432,12 +428,8 @@ static int launch_trace_jobs(int blocks, size_t lds, hipStream_t s, const TraceA
 
 template <class S, int NC>
 static int launch_trace(int blocks, size_t lds, hipStream_t s, const TraceArgs& a) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    MP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trace<S, NC>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> attr_devs{0};
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_trace<S, NC>), (int)lds));
   hipLaunchKernelGGL((k_trace<S, NC>), dim3(blocks), dim3(256), lds, s, a);
   MP_LAUNCHED("k_trace");
   return MP_OK;

@@ -98,11 +98,14 @@ class LevelStack:
         _lib.check(self._lib.mp_stack_set_tiles(self._handle, int(column_blocks)))
 
     def _current_versions(self):
-        return tuple((w._version, b._version, w.data_ptr(), b.data_ptr())
-                     for layers in self._layers for w, b, _ in layers)
+        # the frequencies are read from the modules each time: update_omegas()
+        # bakes them into the weights and resets them to 1 in place
+        return tuple((w._version, b._version, w.data_ptr(), b.data_ptr(), omega)
+                     for m in self.models for w, b, omega in _linear_layers(m))
 
     @torch.no_grad()
     def _copy_weights(self):
+        self._layers = [_linear_layers(m) for m in self.models]
         for i, layers in enumerate(self._layers):
             ow = ob = 0
             for w, b, omega in layers:

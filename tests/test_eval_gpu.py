@@ -203,3 +203,17 @@ def test_single_column_tiles_in_the_culled_extraction(precision):
     _, counts_b = st.extract_volume(96, [0.05], chunk=4096, out=vol_b)
     assert counts_a == counts_b and counts_a[1] > 0
     assert torch.equal(vol_a, vol_b)
+
+
+def test_forward_after_update_omegas_is_unchanged():
+    # the module's cached stack re-packs the baked weights with the new
+    # frequency (update_omegas rewrites the parameters and resets w0 in place)
+    from mplicits.model import SIREN
+    torch.manual_seed(8)
+    m = SIREN(3, 1, [64, 64], w0=30, ww=30).cuda()
+    x = _points(4096, 9).float().cuda()
+    with torch.no_grad():
+        before = m(x)["model_out"].clone()
+        m.update_omegas(1)
+        after = m(x)["model_out"]
+    assert (after - before).abs().max().item() <= 1e-5
