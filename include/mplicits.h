@@ -118,6 +118,21 @@ int mp_eval(const mp_stack* stack, const float* xyz, int64_t n, int lvl0, int lv
 int mp_extract_volume(mp_stack* stack, int N, int grid_mode, int64_t chunk, const float* deltas,
                       int n_deltas, float* vol, int64_t* level_counts, void* stream);
 
+/* mp_extract_volume without any allocation: the same kernels and results, with
+ * the work lists in a fixed device array of the library
+ * (MP_NOALLOC_SCRATCH_BYTES, held on every GPU the library runs on until the
+ * process ends) instead of the stack's scratch.  Limits: N^3 <=
+ * MP_NOALLOC_MAX_POINTS (N <= 256) and (levels - 1) * ceil(N^3 / chunk) <= 2^20;
+ * larger calls return MP_EINVAL before any launch.  Still a host-sync point
+ * (band counts are read back).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two threads) race.
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under graph capture, serving loops, harnesses that forbid hipMalloc). */
+#define MP_NOALLOC_MAX_POINTS (1 << 24)
+#define MP_NOALLOC_SCRATCH_BYTES (256 + (1 << 20) + 8 * (size_t)MP_NOALLOC_MAX_POINTS)
+int mp_extract_volume_noalloc(mp_stack* stack, int N, int grid_mode, int64_t chunk, const float* deltas,
+                              int n_deltas, float* vol, int64_t* level_counts, void* stream);
+
 /* Batched band-culled extraction of many stacks (uneven widths, levels and
  * resolutions; every stack in the same precision).  The reference extracts
  * one model per create_mesh_multistage call (i3d/meshing.py:135-264); this

@@ -122,6 +122,9 @@ SIGNATURES = {
     "mp_extract_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                          c_float_p, ctypes.c_int, ctypes.c_void_p, c_int64_p,
                                          ctypes.c_void_p]),
+    "mp_extract_volume_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                                 c_float_p, ctypes.c_int, ctypes.c_void_p, c_int64_p,
+                                                 ctypes.c_void_p]),
     "mp_grid_coords": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_void_p, ctypes.c_void_p]),
     "mp_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
