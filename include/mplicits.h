@@ -164,7 +164,13 @@ void mp_extract_batch_destroy(mp_batch* plan);
  * (gen_mc_coordinate_grid, i3d/meshing.py:11-69, columns 0..2). */
 int mp_grid_coords(int N, int grid_mode, int64_t first, int64_t n, float* xyz, void* stream);
 
-/* Multiscale sphere tracing of the whole stack.  Replaces the renderer's
+/* Multiscale sphere tracing of the whole stack.  Step rule per ray: the
+ * renderer's lod-0 phase (step by F_0 - delta_0) until the ray is first inside
+ * the coarse band F_0 < band_scale * delta_0; from then on, inside the band
+ * step by the sum of all levels (a hit when |sum| < hit_eps), outside it by
+ * F_0 (IterateSphereTracing_kernel, kernels.cu:315-398, with its fixed-length
+ * phases switched per ray).  A single level steps by F_0.
+ * Replaces the renderer's
  * MIPplicit::SdfInference tracing loop and shading (renderer/cuda/state.cu:816-1055;
  * InitializeRays_kernel, IterateSphereTracing_kernel and Shade_kernel in
  * renderer/cuda/kernels.cu:122-487) and the Python multiscale_sphere_tracing
@@ -188,7 +194,8 @@ typedef struct mp_trace_params {
   int pixel_center;    /* 1: NDC from pixel centres; 0: the renderer's 2i/W - 1 */
   int flip_y;
   int n_deltas;        /* n_levels - 1 */
-  float deltas[MP_MAX_LEVELS];
+  float deltas[MP_MAX_LEVELS]; /* the band widths of levels 0 .. n_levels - 2; the step rule reads
+                                  deltas[0] only, as the renderer reads delta_lod_0 only */
 } mp_trace_params;
 
 int mp_trace(mp_stack* stack, const float* cams, int n_cams, const mp_trace_params* params,

@@ -32,12 +32,12 @@ struct TraceArgs {
   float* hit_pos;                // [rays][3] (scratch)
   uint32_t* rgba;                // [rays] RGBA8
   float* normals;                // [rays][3] or NULL
-  // Level-by-level launches (eval_levels > 0): the launch evaluates levels
-  // 0 .. eval_levels - 1 and parks, before its step, every ray that needs
-  // level eval_levels; the next launch traces the parked rays.  Each ray gets
-  // the same evaluations and steps as in the single pass (eval_levels 0: every
-  // level, the ray queue, no parking); only the rays that need the wider
-  // levels stay together in the tiles.
+  // Two-launch tracing (eval_levels > 0): the first launch (eval_levels 1)
+  // evaluates the coarse level and parks, before its step, every ray that
+  // enters the coarse band; the second (eval_levels = n_levels) traces the
+  // parked rays on all levels.  Each ray gets the same evaluations and steps
+  // as in the single pass (eval_levels 0: every level, the ray queue, no
+  // parking); only the rays that need the residual levels share tiles.
   int eval_levels;
   const ParkedRay* park_in;               // rays of this launch (NULL: the ray queue)
   const unsigned long long* n_park_in;    // their count

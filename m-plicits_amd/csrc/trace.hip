@@ -8,20 +8,24 @@
 // hit or left the scene are written out and their slots refilled from the
 // wave's pool of ray ids (reserved from a global ray counter 64 at a time), so
 // the matrix cores only ever see live rays.  The coarse level
-// is evaluated for every live ray; a residual level only when some ray of the
-// tile is inside the band of the partial sum before it.  A stack with residual
-// levels is traced level by level: every ray on the coarse level until it
-// enters the coarse band, then the rays parked there on the levels up to the
-// next one until they enter that level's band, and so on, so that far-field
+// is evaluated for every live ray; the residual levels only when some ray of
+// the tile is inside the coarse band.  A stack with residual levels is traced
+// in two launches: every ray on the coarse level until it enters the coarse
+// band (parked there), then the parked rays on all levels, so that far-field
 // rays (coarse only) and near-surface rays (every level) do not share tiles.
 //
-// Step rule (the renderer's, with a hit test):
-//   F_0 = f_0(p).  Outside the coarse band (F_0 >= s * delta_0, s = band_scale,
-//   1.4 in the renderer, kernels.cu:335,374) step by F_0 - delta_0.  Inside it,
-//   F_i = F_{i-1} + f_i(p) for every residual level i whose band the partial sum
-//   is in (|F_{i-1}| < s * delta_{i-1}) and step by the full sum.  A ray hits
-//   when |F| < hit_eps for the evaluated sum, and misses when it leaves the
-//   bounding sphere or runs out of steps.
+// Step rule: IterateSphereTracing_kernel's (kernels.cu:315-398) with its
+// fixed-length phases replaced by a per-ray switch, plus a hit test.
+//   F_0 = f_0(p), banded = F_0 < s * delta_0 (s = band_scale, 1.4 in the
+//   renderer, kernels.cu:335,374).
+//   lod-0 phase (kernels.cu:331-336,369-373), until the ray is first banded:
+//   step by F_0 - delta_0.
+//   residual phase (kernels.cu:337-358,374-394), from then on: banded, step
+//   by F = F_0 + f_1(p) + ... + f_{L-1}(p) (every residual level; the
+//   renderer's delta_lod_1 is unused); otherwise step by F_0.
+//   A banded ray hits when |F| < hit_eps; a ray misses when it leaves the
+//   bounding sphere or runs out of steps.  One level: step by F_0, hit when
+//   |F_0| < hit_eps.
 // Rays start where they enter the sphere of radius bound_radius
 // (InitializeRays_kernel, kernels.cu:400-487).
 #include "trace_core.h"
@@ -58,13 +62,16 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
   int64_t rid[NC];
   float p[NC][3], d[NC][3], t[NC], t0[NC];
   int steps[NC];
-  // a parked ray's first evaluation repeats, at the same point, the levels the
-  // launch before evaluated (0 .. n_eval - 2): those are not counted again
+  // a parked ray's first evaluation repeats, at the same point, the coarse
+  // level the launch before evaluated: that one is not counted again
   bool fresh[NC];
+  // the ray has been in the coarse band (its lod-0 phase is over)
+  bool entered[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     rid[c] = -1;
     fresh[c] = false;
+    entered[c] = false;
   }
   const float R2 = a.bound_radius * a.bound_radius * 1.0001f;
   unsigned long long evals[MP_MAX_LEVELS] = {0, 0, 0, 0};
@@ -110,6 +117,7 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
             steps[c] = r.steps;
             rid[c] = id;
             fresh[c] = true;
+            entered[c] = true;  // parked where it entered the coarse band
           } else {
             bool inside;
             ray_init(a, nid, p[c], d[c], t0[c], inside);
@@ -117,6 +125,7 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
               rid[c] = nid;
               t[c] = 0.f;
               steps[c] = 0;
+              entered[c] = false;
             } else {
               // the ray never meets the bounding sphere: a miss
               a.depth[nid] = -1.0f;
@@ -138,6 +147,7 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
       d[c][0] = __shfl(d[c][0], src); d[c][1] = __shfl(d[c][1], src); d[c][2] = __shfl(d[c][2], src);
       t[c] = __shfl(t[c], src); t0[c] = __shfl(t0[c], src);
       steps[c] = __shfl(steps[c], src);
+      entered[c] = __shfl((int)entered[c], src) != 0;
     }
     bool live = false;
 #pragma unroll
@@ -180,9 +190,9 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
         for (int c = 0; c < NC; ++c) {
           if (need[c]) {
             Fsum[c] += r[c];
-            if (lane < S::ROWS && !(fresh[c] && l < n_eval - 1)) evals[l]++;
+            if (lane < S::ROWS) evals[l]++;
           }
-          need[c] = need[c] && l + 1 < a.n_levels && fabsf(Fsum[c]) < a.band_scale * a.deltas[l];
+          need[c] = need[c] && l + 1 < a.n_levels;
         }
       }
     }
@@ -221,11 +231,12 @@ __device__ __forceinline__ void trace_rays(const TraceArgs& a, typename S::Frag*
         s = F[c];
         hit = fabsf(F[c]) < a.hit_eps;
       } else if (!banded) {
-        s = F[c] - a.deltas[0];
+        s = entered[c] ? F[c] : F[c] - a.deltas[0];
         hit = false;
       } else {
         s = Fsum[c];
         hit = fabsf(Fsum[c]) < a.hit_eps;
+        entered[c] = true;
       }
       bool done = hit;
       if (!hit) {
@@ -463,9 +474,7 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   const size_t off_ids = off_pos + (size_t)total * 12;
   const size_t off_cpos = (off_ids + (size_t)total * 4 + 255) & ~(size_t)255;
   const size_t off_grad = off_cpos + (size_t)total * 12;
-  // | second parked-ray list [total] (stacks of three or more levels)
-  const size_t off_park2 = (off_grad + (size_t)total * 12 + 255) & ~(size_t)255;
-  int rc = ensure_scratch(st, st->n_levels >= 3 ? off_park2 + (size_t)total * 24 : off_grad + (size_t)total * 12);
+  int rc = ensure_scratch(st, off_grad + (size_t)total * 12);
   if (rc != MP_OK) return rc;
   char* base = reinterpret_cast<char*>(st->scratch);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base);
@@ -473,7 +482,6 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   uint32_t* ids = reinterpret_cast<uint32_t*>(base + off_ids);
   float* cpos = reinterpret_cast<float*>(base + off_cpos);
   float* grad = reinterpret_cast<float*>(base + off_grad);
-  char* park2 = st->n_levels >= 3 ? base + off_park2 : nullptr;
 
   TraceArgs a{};
   a.cams = cams;
@@ -528,22 +536,29 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
     rc = launch(a);
     if (rc != MP_OK) return rc;
   } else {
-    // one launch per level: launch k evaluates levels 0 .. k - 1 and parks the
-    // rays that need level k for launch k + 1.  Two parked lists alternate
-    // (24 B per ray each; a ray is parked at most once per launch): the first
-    // takes the compacted-hit and gradient regions, written only after tracing.
+    // two launches: the first traces the coarse level and parks every ray
+    // where it enters the coarse band (before its step); the second traces
+    // the parked rays on all levels.  The parked list (24 B per ray, each ray
+    // parked at most once) takes the compacted-hit and gradient regions,
+    // written only after tracing.
     static_assert(sizeof(ParkedRay) == 24, "parked rays must fit the 24 B per ray of cpos + grad");
-    ParkedRay* lists[2] = {reinterpret_cast<ParkedRay*>(cpos), reinterpret_cast<ParkedRay*>(park2)};
-    for (int k = 1; k <= st->n_levels; ++k) {
-      a.eval_levels = k;
-      a.park_in = k > 1 ? lists[k % 2] : nullptr;            // written by launch k - 1
-      a.n_park_in = k > 1 ? &ctr[8 + k - 1] : nullptr;
-      a.park_out = k < st->n_levels ? lists[(k + 1) % 2] : nullptr;
-      a.n_park_out = k < st->n_levels ? &ctr[8 + k] : nullptr;
-      a.next_ray = k > 1 ? &ctr[16 + k] : &ctr[0];
-      rc = launch(a);
-      if (rc != MP_OK) return rc;
-    }
+    ParkedRay* parked = reinterpret_cast<ParkedRay*>(cpos);
+    a.eval_levels = 1;
+    a.park_in = nullptr;
+    a.n_park_in = nullptr;
+    a.park_out = parked;
+    a.n_park_out = &ctr[9];
+    a.next_ray = &ctr[0];
+    rc = launch(a);
+    if (rc != MP_OK) return rc;
+    a.eval_levels = st->n_levels;
+    a.park_in = parked;
+    a.n_park_in = &ctr[9];
+    a.park_out = nullptr;
+    a.n_park_out = nullptr;
+    a.next_ray = &ctr[17];
+    rc = launch(a);
+    if (rc != MP_OK) return rc;
   }
   hipLaunchKernelGGL(k_collect_hits, dim3((unsigned)((total + COLLECT_PER_BLOCK - 1) / COLLECT_PER_BLOCK)),
                      dim3(256), 0, s, depth,

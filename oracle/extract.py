@@ -18,28 +18,44 @@ from . import grid as ogrid
 from . import siren as osiren
 
 
-def sample_chunk(level_layers, xyz, deltas, dtype=np.float64):
-    """Restated loop body for one chunk; level_layers are oracle layer lists."""
-    s = osiren.forward(level_layers[0], xyz).astype(dtype)
-    for i in range(1, len(level_layers)):
+def _level_fn(level):
+    if callable(level):
+        return level
+    return lambda x: osiren.forward(level, x)
+
+
+def sample_chunk(level_layers, xyz, deltas, dtype=np.float64, return_masks=False):
+    """Restated loop body for one chunk (i3d/meshing.py:212-236).
+
+    level_layers: per level an oracle layer list or a callable f(x [n, 3]) -> [n].
+    The band of level i is taken on the sum so far (meshing.py:225 masks the
+    updated samples tensor).  With return_masks, also the samples each residual
+    level was added to ([n] bool per level 1..L-1)."""
+    fns = [_level_fn(lv) for lv in level_layers]
+    s = np.asarray(fns[0](xyz)).astype(dtype)
+    masks = [np.zeros(len(s), bool) for _ in fns[1:]]
+    for i in range(1, len(fns)):
         if deltas:
             m = np.abs(s) < np.float32(deltas[i - 1])
             if not m.any():
                 break
-            s[m] = s[m] + osiren.forward(level_layers[i], xyz[m]).astype(dtype)
+            s[m] = s[m] + np.asarray(fns[i](xyz[m])).astype(dtype)
         else:
-            s = s + osiren.forward(level_layers[i], xyz).astype(dtype)
-    return s
+            m = np.ones(len(s), bool)
+            s = s + np.asarray(fns[i](xyz)).astype(dtype)
+        masks[i - 1] = m
+    return (s, masks) if return_masks else s
 
 
-def sample_chunks(level_layers, N, deltas, max_batch, chunk_ids, mode="reference"):
-    """{chunk id: (indices, values)} for the requested chunks (FP64 evaluation)."""
+def sample_chunks(level_layers, N, deltas, max_batch, chunk_ids, mode="reference", return_masks=False):
+    """{chunk id: (indices, values[, masks])} for the requested chunks (FP64 evaluation)."""
     out = {}
     total = N ** 3
     for c in chunk_ids:
         idx = np.arange(c * max_batch, min((c + 1) * max_batch, total), dtype=np.int64)
         xyz = ogrid.coords(N, idx, mode).astype(np.float64)
-        out[c] = (idx, sample_chunk(level_layers, xyz, deltas))
+        r = sample_chunk(level_layers, xyz, deltas, return_masks=return_masks)
+        out[c] = (idx,) + (r if return_masks else (r,))
     return out
 
 

@@ -37,3 +37,36 @@ def param_grads(layers, x, loss_of):
     params = [p for wb in layers for p in wb]
     loss = loss_of(layers, x)
     return torch.autograd.grad(loss, params)
+
+
+def config_loss(fine, medium, coarse, x, f_t, n_t):
+    """The benchmark's training loss on the level sum F = f + r + c (only the
+    fine level trainable; r, c and their gradients detached, as
+    loss_add_detail_fine treats them):
+        mean|F - f_t| + mean||grad F| - 1| + mean(1 - cos(grad F, n_t)).
+    f_t [n], n_t [n, 3]; returns a scalar tensor."""
+    f, gf = level_value_grad(fine, x, True)
+    r, gr = level_value_grad(medium, x, False)
+    c, gc = level_value_grad(coarse, x, False)
+    F = f + r.detach() + c.detach()
+    G = gf + gr.detach() + gc.detach()
+    gn = G.norm(dim=-1)
+    cos = torch.nn.functional.cosine_similarity(G, n_t, dim=-1)
+    return (F - f_t).abs().mean() + (gn - 1).abs().mean() + (1 - cos).mean()
+
+
+def true_sdf(layers, x, gt_sdf, gt_normals):
+    """The coarse trainer's loss (true_sdf, i3d/loss_functions.py:123-163) on one
+    level given as FP64 (W, b) tensors: 3e3 mean[f^2 | gt = 0] + 2e2
+    mean[(gt - f)^2 | gt != 0] + 1e2 mean[1 - cos(grad f, n) | gt = 0] + 5e1
+    mean[(|grad f| - 1)^2] (every mean over all points, zeros elsewhere)."""
+    f, g = level_value_grad(layers, x, True)
+    on = gt_sdf == 0
+    zero = torch.zeros_like(f)
+    sdf_on = torch.where(on, f ** 2, zero)
+    sdf_off = torch.where(~on, (gt_sdf - f) ** 2, zero)
+    cos = torch.nn.functional.cosine_similarity(g, gt_normals, dim=-1)[..., None]
+    normal = torch.where(on, 1 - cos, torch.zeros_like(gt_sdf))
+    eik = ((g.norm(dim=-1) - 1.0) ** 2).unsqueeze(-1)
+    return {"sdf_on_surf": sdf_on.mean() * 3e3, "sdf_off_surf": sdf_off.mean() * 2e2,
+            "normal_constraint": normal.mean() * 1e2, "grad_constraint": eik.mean() * 5e1}

@@ -13,6 +13,7 @@ from mplicits.synthetic import orbit_cameras, seeded_siren
 from mplicits.tracer import trace
 from oracle import siren as osiren
 from oracle import trace as otrace
+from checks import check_trace_against_oracle as _check_against_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -58,28 +59,24 @@ def test_plane_depth_and_normals_closed_form(precision):
     assert out["hits"] == int(got_hit.sum())
 
 
-def test_stack_matches_oracle_step_rule():
+@pytest.mark.parametrize("precision,prec_err", [("fp32", 0.0), ("bf16", 1e-3), ("fp16", 3e-4)])
+def test_stack_matches_oracle_step_rule(precision, prec_err):
     coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
     levels = [coarse, seeded_siren([128] * 3, 60.0, seed=42, last_scale=0.1).cuda()]
     deltas = [0.05]
     cams = orbit_cameras(2, seed=9)
     W, H = 64, 48
-    out = trace(LevelStack(levels, "fp32"), cams, W, H, deltas=deltas)
-    depth = out["depth"].cpu().numpy().reshape(-1)
-    rng = np.random.default_rng(0)
-    ids = rng.choice(len(depth), 600, replace=False)
-    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
-    layers = [osiren.layers_from_model(m) for m in levels]
-    hit, t, _ = otrace.trace(layers, deltas, P[inside], D[inside])
-    ref_depth = np.full(len(ids), -1.0)
-    ref_depth[np.nonzero(inside)[0][hit]] = (t0[inside] + t)[hit]
-    got = depth[ids]
-    agree = (got >= 0) == (ref_depth >= 0)
-    assert agree.mean() > 0.97, agree.mean()
-    both = (got >= 0) & (ref_depth >= 0)
-    assert both.sum() > 50
-    assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
+    out = trace(LevelStack(levels, precision), cams, W, H, deltas=deltas)
+    _check_against_oracle(out, levels, deltas, cams, W, H, 600, 0, prec_err)
     assert out["evals"][0] > 0 and out["evals"][1] > 0
+
+
+def test_single_level_matches_oracle_step_rule():
+    coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
+    cams = orbit_cameras(2, seed=10)
+    W, H = 64, 48
+    out = trace(LevelStack([coarse], "fp32"), cams, W, H, deltas=[])
+    _check_against_oracle(out, [coarse], [], cams, W, H, 600, 4)
 
 
 @pytest.mark.parametrize("flags,iters", [(["-iters=20,5,5", "-delta=0.05", "-normal_lod=2"], "20/5/5"),
@@ -135,41 +132,29 @@ def test_rays_that_miss_the_bounding_sphere(n_levels):
 
 
 def test_three_level_stack_matches_oracle_step_rule():
-    """Three levels (the tracer's level-by-level launches, a 512-wide fine
-    level) against the FP64 restatement of the step rule."""
+    """Three levels (coarse launch, then the parked rays on every level; a
+    512-wide fine level) against the FP64 restatement of the step rule."""
     coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
     levels = [coarse, seeded_siren([128] * 3, 60.0, seed=42, last_scale=0.1).cuda(),
               seeded_siren([512] * 3, 90.0, seed=44, last_scale=0.1).cuda()]
     deltas = [0.05, 0.02]
     cams = orbit_cameras(2, seed=11)
     W, H = 64, 48
-    out = trace(LevelStack(levels, "fp32"), cams, W, H, deltas=deltas)
-    depth = out["depth"].cpu().numpy().reshape(-1)
-    rng = np.random.default_rng(1)
-    ids = rng.choice(len(depth), 1000, replace=False)  # the surface covers ~8% of the frame
-    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
-    layers = [osiren.layers_from_model(m) for m in levels]
-    hit, t, _ = otrace.trace(layers, deltas, P[inside], D[inside])
-    ref_depth = np.full(len(ids), -1.0)
-    ref_depth[np.nonzero(inside)[0][hit]] = (t0[inside] + t)[hit]
-    got = depth[ids]
-    agree = (got >= 0) == (ref_depth >= 0)
-    assert agree.mean() > 0.97, agree.mean()
-    both = (got >= 0) & (ref_depth >= 0)
-    assert both.sum() > 50
-    assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
-    assert all(e > 0 for e in out["evals"]) and out["evals"][0] > out["evals"][1] > 0
+    for precision, prec_err in (("fp32", 0.0), ("bf16", 1e-3)):
+        out = trace(LevelStack(levels, precision), cams, W, H, deltas=deltas)
+        # the surface covers ~8% of the frame
+        _check_against_oracle(out, levels, deltas, cams, W, H, 1000, 1, prec_err)
+        assert all(e > 0 for e in out["evals"]) and out["evals"][0] > out["evals"][1] == out["evals"][2]
 
 
 def test_four_level_stack_matches_oracle_step_rule():
-    """Four levels (the most a stack holds): the third launch parks into the
-    list the second one read, the fourth traces it."""
+    """Four levels (the most a stack holds)."""
     coarse = from_pth(osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")).cuda()
     levels = [coarse, seeded_siren([128] * 3, 60.0, seed=45, last_scale=0.1),
               seeded_siren([64] * 2, 90.0, seed=46, last_scale=0.1),
               seeded_siren([128] * 2, 120.0, seed=47, last_scale=0.05)]
     # residuals without output bias: stacked constant offsets would move the
-    # zero of the sum out of the nested bands, and no ray would hit
+    # zero of the sum out of the coarse band, and no ray would hit
     with torch.no_grad():
         for m in levels[1:]:
             m.net[-1][0].bias.zero_()
@@ -178,18 +163,5 @@ def test_four_level_stack_matches_oracle_step_rule():
     cams = orbit_cameras(2, seed=13)
     W, H = 64, 48
     out = trace(LevelStack(levels, "fp32"), cams, W, H, deltas=deltas)
-    depth = out["depth"].cpu().numpy().reshape(-1)
-    rng = np.random.default_rng(2)
-    ids = rng.choice(len(depth), 1000, replace=False)  # the surface covers ~8% of the frame
-    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
-    layers = [osiren.layers_from_model(m) for m in levels]
-    hit, t, _ = otrace.trace(layers, deltas, P[inside], D[inside])
-    ref_depth = np.full(len(ids), -1.0)
-    ref_depth[np.nonzero(inside)[0][hit]] = (t0[inside] + t)[hit]
-    got = depth[ids]
-    agree = (got >= 0) == (ref_depth >= 0)
-    assert agree.mean() > 0.97, agree.mean()
-    both = (got >= 0) & (ref_depth >= 0)
-    assert both.sum() > 50
-    assert np.median(np.abs(got[both] - ref_depth[both])) < 1e-4
+    _check_against_oracle(out, levels, deltas, cams, W, H, 1000, 2)
     assert len(out["evals"]) == 4 and all(e > 0 for e in out["evals"])
