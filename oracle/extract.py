@@ -92,3 +92,45 @@ def torch_sampling_loop(level_layers, N, deltas, max_batch, chunk_ids=None, mode
                     s += osiren.torch_forward(level_layers[i], x).squeeze(-1)
             out[c] = (idx, s.numpy())
     return out, counts
+
+
+def sampling_loop_full(level_layers, N, deltas, max_batch, device, dtype=torch.float64, mode="reference",
+                       batch=1 << 20):
+    """The whole-grid result of the chunked loop (i3d/meshing.py:212-236) in
+    torch (FP64 by default) on `device`: the same per-chunk semantics as
+    sample_chunk, evaluated level by level over the grid.
+
+    level_layers: per level [(W, b)] torch tensors of `dtype` on `device`
+    (baked).  Returns (values [N^3], masks: per residual level the [N^3] bool
+    samples it was added to)."""
+    total = N ** 3
+    n_chunks = (total + max_batch - 1) // max_batch
+
+    def level_at(layers, idx):
+        # coordinates in float32 on the CPU (bit-identical to ogrid.coords),
+        # the level on the device
+        out = torch.empty(len(idx), dtype=dtype, device=device)
+        for i in range(0, len(idx), batch):
+            x = ogrid.coords_torch(N, idx[i:i + batch].cpu(), mode).to(device=device, dtype=dtype)
+            out[i:i + batch] = osiren.torch_forward(layers, x)[:, 0]
+        return out
+
+    with torch.no_grad():
+        s = level_at(level_layers[0], torch.arange(total, device=device))
+        masks = []
+        alive = torch.ones(n_chunks, dtype=torch.bool, device=device)
+        chunk_of = lambda i: torch.div(i, max_batch, rounding_mode="floor")  # noqa: E731
+        for li in range(1, len(level_layers)):
+            if deltas:
+                raw = s.abs() < float(np.float32(deltas[li - 1]))
+                hit = torch.zeros(n_chunks, dtype=torch.bool, device=device)
+                hit[chunk_of(torch.nonzero(raw).flatten())] = True
+                alive = alive & hit
+                m = raw & alive.repeat_interleave(max_batch)[:total]
+            else:
+                m = torch.ones(total, dtype=torch.bool, device=device)
+            idx = torch.nonzero(m).flatten()
+            if len(idx):
+                s[idx] += level_at(level_layers[li], idx)
+            masks.append(m)
+    return s, masks

@@ -43,3 +43,25 @@ def coords(N, idx, mode="reference"):
     out[:, 1] = (vy * h) + one
     out[:, 2] = (vz * h) + one
     return out
+
+
+def coords_torch(N, idx, mode="reference"):
+    """coords() in torch on idx's device (int64 tensor of linear indices): the
+    same float32 operations in the same order."""
+    import torch
+    idx = idx.to(torch.int64)
+    fN = torch.tensor(float(N), dtype=torch.float32, device=idx.device)
+    if mode == "reference":
+        vz = (idx % N).to(torch.float32)
+        t = idx.to(torch.float32) / fN
+        vy = torch.fmod(t, fN)
+        vx = torch.fmod(t / fN, fN)
+    elif mode == "lattice":
+        vx = (idx // (N * N)).to(torch.float32)
+        vy = ((idx // N) % N).to(torch.float32)
+        vz = (idx % N).to(torch.float32)
+    else:
+        raise ValueError(mode)
+    h = torch.tensor(float(voxel_size32(N)), dtype=torch.float32, device=idx.device)
+    one = torch.tensor(-1.0, dtype=torch.float32, device=idx.device)
+    return torch.stack([(vx * h) + one, (vy * h) + one, (vz * h) + one], 1)

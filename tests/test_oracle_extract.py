@@ -98,3 +98,25 @@ def test_masks_name_the_samples_each_level_received():
     np.testing.assert_array_equal(m2, [True, False, False, True])
     _, (m1, m2) = oextract.sample_chunk([f0, f1, f2], _pts(4), [0.001, 0.5], return_masks=True)
     assert not m1.any() and not m2.any()  # the loop broke at level 1
+
+
+def test_whole_grid_loop_equals_the_chunk_loop():
+    # the torch whole-grid restatement (the benchmark's FP64 checker) against
+    # the per-chunk restatement, including the break and the updated-sum masks
+    rng = np.random.default_rng(4)
+    mk = lambda w, s: [(rng.normal(size=(w, 3)) * s, rng.normal(size=w)), (rng.normal(size=(1, w)) * 0.05,  # noqa
+                                                                             rng.normal(size=1) * 0.02)]
+    layers = [mk(16, 3.0), mk(8, 5.0), mk(8, 7.0)]
+    layers[0][1] = (layers[0][1][0] * 4, np.array([0.05]))
+    N, chunk = 12, 100
+    for deltas in ([], [0.05, 0.02], [0.004, 0.2]):
+        ref = oextract.sample_chunks(layers, N, deltas, chunk, range((N ** 3 + chunk - 1) // chunk),
+                                     return_masks=True)
+        tl = [[(torch.from_numpy(w), torch.from_numpy(b)) for w, b in lay] for lay in layers]
+        s, masks = oextract.sampling_loop_full(tl, N, deltas, chunk, "cpu", batch=333)
+        for c, (idx, v, m) in ref.items():
+            np.testing.assert_allclose(s.numpy()[idx], v, rtol=0, atol=1e-13)
+            for a, b in zip(masks, m):
+                np.testing.assert_array_equal(a.numpy()[idx], b)
+        if deltas:
+            assert 0 < masks[0].sum() < N ** 3

@@ -56,3 +56,14 @@ def test_bit_exact_against_torch_semantics_512_regions():
 def test_reference_grid_is_not_the_lattice():
     idx = np.arange(1, 64)
     assert not np.array_equal(ogrid.coords(4, idx), ogrid.coords(4, idx, "lattice"))
+
+
+def test_torch_restatement_is_bit_identical():
+    import torch
+    for N in (7, 64, 512):
+        idx = np.unique(np.concatenate([np.arange(min(N ** 3, 5000)), np.arange(N ** 3 - 100, N ** 3),
+                                        np.random.default_rng(N).integers(0, N ** 3, 5000)]))
+        for mode in ("reference", "lattice"):
+            a = ogrid.coords(N, idx, mode)
+            b = ogrid.coords_torch(N, torch.from_numpy(idx), mode).numpy()
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (N, mode)
