@@ -57,7 +57,7 @@ SDF_BOUND = 1e-4
 # HBM bytes of one launch of the dominant kernel (coarse level over the
 # grid) per precision, from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
 # copy-calibrated (profiles/extract512_hbm_counters.md)
-TRAFFIC_BYTES = {"bf16": (2774 + 524288) * 1024}
+TRAFFIC_BYTES = {"bf16": (2774 + 524288) * 1024, "bf16x3": (5048 + 555042) * 1024}
 
 
 def level_flops(hidden):
@@ -86,6 +86,16 @@ def timed(stack, vol, steps, warmup):
         counts, nv, nf = extraction_step(stack, vol)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3 / steps, counts, nv, nf
+
+
+def bench_extract(levels, precisions, steps, warmup):
+    """The extraction step alone (profiling runs): ms per step per precision."""
+    vol = torch.empty(N, N, N, device="cuda")
+    out = {}
+    for prec in precisions:
+        ms, counts, nv, nf = timed(LevelStack(levels, prec), vol, steps, warmup)
+        out[prec] = {"ms": round(ms, 3), "counts": counts, "faces": nf}
+    return out
 
 
 def mesh_of(vol):
@@ -505,7 +515,7 @@ def main():
                     help="comma list of paths to time (profiling runs use one)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB tensor (HBM counter calibration for rocprofv3 --pmc)")
-    ap.add_argument("--only", choices=["trace", "train", "train_data", "render", "batch"], default=None,
+    ap.add_argument("--only", choices=["extract", "trace", "train", "train_data", "render", "batch"], default=None,
                     help="run one of the side figures alone and print it as JSON (profiling runs)")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -525,7 +535,8 @@ def main():
     levels = load_levels()
     if args.only:
         precs = args.precisions.split(",") if args.precisions != "bf16,bf16x3,fp16,fp32" else None
-        fig = {"trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"], check=not args.no_cpu_baseline),
+        fig = {"extract": lambda: bench_extract(levels, precs or ["bf16x3"], args.steps, args.warmup),
+               "trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"], check=not args.no_cpu_baseline),
                "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps),
                "train_data": lambda: bench_train_data(levels),
                "render": lambda: bench_render(tuple(precs or ("fp16", "fp32")), frames=args.steps * 20),
