@@ -266,69 +266,33 @@ int mp_trace_batch_run(mp_trace_batch* plan, void* stream);
 int mp_trace_batch_stats(const mp_trace_batch* plan, int64_t* stats, void* stream);
 void mp_trace_batch_destroy(mp_trace_batch* plan);
 
-/* Elementwise stages of the residual-level training step (value + T input
- * tangents per point, T <= 3), replacing the autograd graph of
- * loss_add_detail_fine (i3d/loss_functions.py:327-394) with its
- * create_graph gradient (i3d/diff_operators.py:259-265) and double backward
- * (experiment_scripts/train_sdf_on_neighborhood_fine.py:220-252).  Activation
- * blocks are [n][1+T][W]; Z (pre-activations) is float32, X / Abar (GEMM
- * operands) have element type x_dtype / a_dtype: 0 float32, 1 bfloat16,
- * 2 float16.  The GEMMs between the stages are issued by the caller.
- *   mp_train_first:   Z = [W0 x + b0; W0 v_t], X = [sin; cos * tangent]   (layer 0)
- *   mp_train_act:     Z[:, 0] += b, X = [sin; cos * tangent]              (hidden layer)
- *   mp_train_act_bwd: Abar = [cos pbar - sin sum_t ad_t pdbar_t; cos pdbar_t],
- *                     abar_val [n][W] (float32, may be NULL) = value row
- *   mp_train_seed:    Pbar = [s w_L; g_t w_L] (s or g may be NULL = 0) */
-int mp_train_first(const float* x, const float* v, const float* W0, const float* b0, int64_t n, int W,
-                   int n_tangents, float* Z, void* X, int x_dtype, void* Xb, void* stream);
-int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype, void* Xb,
+/* The training step of one level (the trainable residual level of
+ * experiment_scripts/train_sdf_on_neighborhood_fine.py:220-252) in two calls,
+ * replacing the forward, the create_graph input gradient
+ * (i3d/diff_operators.py:259-265) and the double backward that autograd runs
+ * for loss_add_detail_fine (i3d/loss_functions.py:327-394).  Every product
+ * runs on the library's matrix-core GEMMs in the stack's precision (FP32,
+ * BF16 or FP16 operands, FP32 accumulation; MP_BF16X3 is refused); the level
+ * needs >= 2 sine layers of width <= 512.
+ *   mp_train_fwd: f [n] and grad [n][3] (either may be NULL) of level `level`
+ *     at points x [n][3] (device).  keep != 0: the layer activations stay in
+ *     the stack's training workspace for the next mp_train_bwd of the same
+ *     level, x pointer and n (dropped by mp_stack_update or another
+ *     mp_train_fwd).
+ *   mp_train_bwd: dparams (device, mp_train_param_count floats: every layer's
+ *     weights [out][in] in order, then every bias, the .bin order of
+ *     mp_level_desc) = the gradient of sum_p [dL_df[p] f_p + dL_dgrad[p] .
+ *     grad f_p] with respect to the level's frequency-baked weights and
+ *     biases; dL_df [n] or dL_dgrad [n][3] may be NULL (zero).  The layer
+ *     activations are recomputed unless kept by mp_train_fwd.
+ * Both allocate the training workspace on first use (about 16 n W (2 H - 1)
+ * bytes for H sine layers of width W in 16 bit, up to twice that in FP32) and
+ * are otherwise asynchronous on the stream. */
+int64_t mp_train_param_count(const mp_stack* stack, int level);
+int mp_train_fwd(mp_stack* stack, int level, const float* x, int64_t n, float* f, float* grad, int keep,
                  void* stream);
-int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
-                     int a_dtype, float* abar_val, void* Ab, void* stream);
-/* Xb / Ab (may be NULL): a second, K-blocked copy of X / Abar for
- * mp_train_dw (16-bit, three tangents, even n, W % 4 == 0): element
- * (row r = 4 p + t, unit u) at ((r / 8) * W + u) * 8 + r % 8.
- * mp_train_dw: split-K weight gradient dW [Wout][Win] = sum_rows A[r][i] X[r][j]
- * from the blocked copies (rows = 4 n, a multiple of 64; widths multiples of
- * 256): part [mp_train_dw_chunks(rows, Wout, Win)][Wout][Win] FP32 partial
- * sums, to be added by the caller. */
-int mp_train_dw_chunks(int64_t rows, int Wout, int Win);
-int mp_train_dw(const void* Ab, const void* Xb, int64_t rows, int Wout, int Win, int dtype, float* part,
-                void* stream);
-int mp_train_seed(const float* s, const float* g, const float* w_last, int64_t n, int W, int n_tangents,
-                  float* Pbar, void* stream);
-/* Output layer (W -> 1) from the last sine layer's float32 pre-activations Z:
- * forward f [n] = w_L . sin a + b_L, fd [n][T] = w_L . (cos a ad_t), with
- * b_sine (may be NULL) = the last sine layer's bias, added to Z's value row in
- * place first (so that layer needs no mp_train_act: its X is never a GEMM
- * operand); backward Pbar = [s w_L; g_t w_L] and per-block partial sums
- * dwl_part [mp_train_out_bwd_blocks(n)][W] of
- * dw_L = sum s sin a + sum_t g_t cos a ad_t. */
-int mp_train_out_fwd(float* Z, const float* b_sine, const float* w_last, const float* b_last, int64_t n, int W,
-                     int n_tangents, float* f, float* fd, void* stream);
-int64_t mp_train_out_bwd_blocks(int64_t n);
-/* Output layer and last sine layer backward in one pass (W a multiple of 4,
- * at most 512; Pbar is never stored): Abar (and its blocked copy Ab, may be
- * NULL) as mp_train_act_bwd computes it from Pbar = [s w_L; g_t w_L], and
- * part [mp_train_out_bwd_blocks(n)][2][W] per-block partial sums of dw_L
- * (row 0) and the sine layer's db = sum abar (row 1). */
-int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
-                         int n_tangents, void* Abar, int a_dtype, void* Ab, float* part, void* stream);
-/* mp_train_act_bwd with the bias gradient as per-block partial sums
- * db_part [mp_train_out_bwd_blocks(n)][W] instead of the [n][W] value rows
- * (W a multiple of 4, at most 512).  p_dtype: element type of Pbar (0 float32,
- * 1 bfloat16, 2 float16; also for mp_train_first_bwd). */
-int mp_train_act_bwd_part(const float* Z, const void* Pbar, int p_dtype, int64_t n, int W, int n_tangents,
-                          void* Abar, int a_dtype, void* Ab, float* db_part, void* stream);
-/* Layer-0 backward fused with its parameter gradients (W a multiple of 4,
- * at most 512): part [mp_train_first_bwd_blocks(n)][W][4] receives per-block
- * partial sums of (dW_0[u][0..2], db_0[u]) = sum over points of
- * abar x + sum_t adbar_t v_t and abar, with abar / adbar as in mp_train_act_bwd. */
-int64_t mp_train_first_bwd_blocks(int64_t n);
-int mp_train_first_bwd(const float* Z, const void* Pbar, int p_dtype, const float* x, const float* v, int64_t n,
-                       int W, int n_tangents, float* part, void* stream);
-int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
-                     int n_tangents, float* Pbar, float* dwl_part, void* stream);
+int mp_train_bwd(mp_stack* stack, int level, const float* x, int64_t n, const float* dL_df, const float* dL_dgrad,
+                 float* dparams, void* stream);
 
 /* Training-data generation (the step before the training kernels).
  * mp_mesh: a triangle mesh for unsigned point-to-mesh distances, replacing

@@ -25,6 +25,15 @@ struct mp_stack {
   Level lv[MP_MAX_LEVELS];
   void* scratch = nullptr;  // extraction / tracing work lists, grown on demand
   size_t scratch_bytes = 0;
+  uint64_t version = 0;     // mp_stack_update calls so far
+  // training workspace (mp_train_fwd / mp_train_bwd), grown on demand, and the
+  // activations the last mp_train_fwd kept in it
+  void* train_ws = nullptr;
+  size_t train_ws_bytes = 0;
+  int kept_level = -1;
+  int64_t kept_n = 0;
+  const float* kept_x = nullptr;
+  uint64_t kept_version = 0;
 };
 
 namespace mp {

@@ -213,6 +213,7 @@ void mp_stack_destroy(mp_stack* st) {
   for (int l = 0; l < st->n_levels; ++l)
     if (st->lv[l].buf) (void)hipFree(st->lv[l].buf);
   if (st->scratch) (void)hipFree(st->scratch);
+  if (st->train_ws) (void)hipFree(st->train_ws);
   delete st;
 }
 
@@ -318,6 +319,7 @@ int mp_stack_set_tiles(mp_stack* st, int column_blocks) {
 
 int mp_stack_update(mp_stack* st, void* stream) {
   MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_stack_update: stack is NULL");
+  ++st->version;  // drops the activations kept for training
   for (int l = 0; l < st->n_levels; ++l) {
     int rc = pack_level(st, st->lv[l], as_stream(stream));
     if (rc != MP_OK) return rc;

@@ -1,1023 +1,548 @@
-// Elementwise stages of the residual-level training step (forward with input
-// tangents and its hand-written backward, including the second-order terms of
-// the input-gradient losses).  Replaces the autograd graph of
-// loss_add_detail_fine (i3d/loss_functions.py:327-394: a forward, a
-// create_graph gradient and a double backward) in the trainers
-// (experiment_scripts/train_sdf_on_neighborhood_fine.py:220-252).
+// The training step of one residual level as two library calls (mp_train_fwd,
+// mp_train_bwd), replacing the autograd graph of loss_add_detail_fine
+// (i3d/loss_functions.py:327-394): a forward, a create_graph input gradient
+// (i3d/diff_operators.py:259-265) and a double backward, as the trainers run
+// it (experiment_scripts/train_sdf_on_neighborhood_fine.py:220-252).
 //
 // Notation per sine layer j (frequency baked into W, b):
 //   a_j = W_j p_{j-1} + b_j,     p_j = sin a_j,
-//   ad_j,t = W_j pd_{j-1},t,     pd_j,t = cos(a_j) ad_j,t       (tangent t = 1..T)
-// with p_{-1} = x and pd_{-1},t = v_t (the tangent directions).  Every
-// activation block is laid out [n][1 + T][W]: row 0 the value, rows 1..T the
-// tangents, so one GEMM over n (1 + T) rows applies W_j to all of them.
-// Backward, given pbar_j (value row) and pdbar_j,t (tangent rows):
+//   ad_j,t = W_j pd_{j-1},t,     pd_j,t = cos(a_j) ad_j,t       (t = 1..3)
+// with p_{-1} = x and pd_{-1},t = e_t (the coordinate directions, so
+// ad_0,t = W_0[.][t]).  Output f = w_out . p_{H-1} + b_out, grad f_t =
+// w_out . pd_{H-1},t.  Rows 4p..4p+3 of every activation block are the value
+// and the three tangents of point p, so one GEMM over 4 n rows applies W_j to
+// all of them.  Backward, given the loss seeds s = dL/df and g = dL/dgrad f:
+//   pbar_{H-1} = s w_out, pdbar_{H-1},t = g_t w_out,
 //   abar_j    = cos(a_j) pbar_j - sin(a_j) sum_t ad_j,t pdbar_j,t
 //   adbar_j,t = cos(a_j) pdbar_j,t
-// then dW_j = [abar; adbar]^T [p_{j-1}; pd_{j-1}], db_j = sum abar_j and
-// [pbar; pdbar]_{j-1} = [abar; adbar] W_j (GEMMs, issued by the host).
+//   dW_j = [abar; adbar]_j^T [p; pd]_{j-1},  db_j = sum abar_j,
+//   [pbar; pdbar]_{j-1} = [abar; adbar]_j W_j.
+//
+// Launches (H sine layers; train_gemm.hip holds the GEMMs and their epilogues):
+//   forward   k_tw_first (layer 0) -> TG_ACT (layers 1..H-2) -> TG_OUT
+//             (layer H-1 and the output layer, reduced per point) -> k_tw_out_sum
+//   backward  TG_OUT_BWD (layer H-1 recomputed with the output-layer backward)
+//             -> for j = H-1..1: TG_PART (dW_j), then TG_ACT_BWD (j >= 2) or
+//             TG_FIRST_BWD (j = 1, layer 0's backward and parameter gradients)
+// Stored between them: the unit-blocked activations X_j (GEMM A operands) and
+// their row-blocked copies Xb_j (weight-gradient operands) of layers
+// 0..H-2, and the FP32 pre-activations Z_j of layers 1..H-2.  Layer 0's and
+// layer H-1's pre-activations are recomputed where they are needed (three
+// FMAs per unit for layer 0; the layer H-1 GEMM in the backward), which
+// halves the HBM traffic of those layers.  Widths are padded to 256 and
+// points to 128 (zero weights, zero seeds), so the GEMMs need no edge
+// handling; the padded units and points contribute nothing.
 #include "common.h"
 #include "mlp_core.h"
+#include "stack.h"
+#include "train_gemm.h"
 
 namespace mp {
+namespace {
 
-__device__ inline void st(float* p, int64_t i, float v) { p[i] = v; }
-__device__ inline void st(__bf16* p, int64_t i, float v) { p[i] = (__bf16)v; }
-__device__ inline void st(_Float16* p, int64_t i, float v) { p[i] = (_Float16)v; }
+constexpr int TW_PAD = 256;
+constexpr int TP_PAD = 128;
+constexpr int MAX_LAYERS = 16;
 
-// Layer 0: Z = [a0; ad0] (float32), X = [sin a0; cos a0 * ad0].
-template <class T>
-__global__ void k_train_first(const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ W0,
-                              const float* __restrict__ b0, int64_t n, int W, int nt, float* __restrict__ Z,
-                              T* __restrict__ X) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * W) return;
-  const int64_t p = i / W;
-  const int u = (int)(i % W);
-  const float w0 = W0[3 * u], w1 = W0[3 * u + 1], w2 = W0[3 * u + 2];
-  const float a = fmaf(w2, x[3 * p + 2], fmaf(w1, x[3 * p + 1], fmaf(w0, x[3 * p], b0[u])));
-  float s, c;
-  sincos_reduced(a, s, c);
-  const int64_t base = p * (1 + nt) * W + u;
-  Z[base] = a;
-  st(X, base, s);
-  for (int t = 0; t < nt; ++t) {
-    const float* vt = v + (p * nt + t) * 3;
-    const float ad = fmaf(w2, vt[2], fmaf(w1, vt[1], w0 * vt[0]));
-    Z[base + (int64_t)(1 + t) * W] = ad;
-    st(X, base + (int64_t)(1 + t) * W, c * ad);
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct TrainPlan {
+  int dtype = 0, esize = 4, H = 0;
+  int w[MAX_LAYERS] = {}, wp[MAX_LAYERS] = {};  // true / padded widths of the sine layers
+  int64_t n = 0, np = 0, rows = 0;
+  int64_t woff[MAX_LAYERS + 1] = {}, boff[MAX_LAYERS + 1] = {}, n_params = 0;
+  // workspace offsets (bytes)
+  size_t x4 = 0, w0 = 0, wl = 0, sg = 0;
+  size_t bh[MAX_LAYERS] = {}, wf[MAX_LAYERS] = {}, wt[MAX_LAYERS] = {};
+  size_t Z[MAX_LAYERS] = {}, X[MAX_LAYERS] = {}, Xb[MAX_LAYERS] = {};
+  size_t A[2] = {}, Ab[2] = {}, part_out = 0, part_col = 0, part_dw = 0, tmp = 0;
+  size_t total = 0;
+  int dw_chunks[MAX_LAYERS] = {};
+};
+
+constexpr int64_t TMP_FLOATS = (int64_t)64 * 512 * 512;  // slice sums of the largest reduction
+
+int plan_training(const mp_stack* st, int level, int64_t n, TrainPlan& p) {
+  MP_REQUIRE(st != nullptr, MP_EINVAL, "training: stack is NULL");
+  MP_REQUIRE(level >= 0 && level < st->n_levels, MP_EINVAL, "training: level %d of a %d-level stack", level,
+             st->n_levels);
+  MP_REQUIRE(n >= 1, MP_EINVAL, "training: n must be >= 1, got %lld", (long long)n);
+  MP_REQUIRE(st->precision != MP_BF16X3, MP_EUNSUPPORTED,
+             "training: three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
+  const mp_stack::Level& L = st->lv[level];
+  p.H = L.n_hidden;
+  MP_REQUIRE(p.H >= 2 && p.H < MAX_LAYERS, MP_EUNSUPPORTED, "training: a level needs 2..%d sine layers, got %d",
+             MAX_LAYERS - 1, p.H);
+  p.dtype = st->precision == MP_BF16 ? 1 : st->precision == MP_FP16 ? 2 : 0;
+  p.esize = p.dtype == 0 ? 4 : 2;
+  p.n = n;
+  p.np = round_up(n, TP_PAD);
+  p.rows = 4 * p.np;
+  int in = 3;
+  for (int k = 0; k < p.H; ++k) {
+    p.w[k] = L.hidden[k];
+    p.wp[k] = (int)round_up(p.w[k], TW_PAD);
+    MP_REQUIRE(p.wp[k] <= 512, MP_EUNSUPPORTED, "training: width %d exceeds 512", p.w[k]);
+    p.woff[k + 1] = p.woff[k] + (int64_t)p.w[k] * in;
+    p.boff[k + 1] = p.boff[k] + p.w[k];
+    in = p.w[k];
   }
-}
-
-// Hidden layer: Z holds W_j X_{j-1} (no bias); adds the bias to the value
-// row in place and writes X_j.
-template <class T>
-__global__ void k_train_act(float* __restrict__ Z, const float* __restrict__ b, int64_t n, int W, int nt,
-                            T* __restrict__ X) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * W) return;
-  const int64_t p = i / W;
-  const int u = (int)(i % W);
-  const int64_t base = p * (1 + nt) * W + u;
-  const float a = Z[base] + b[u];
-  Z[base] = a;
-  float s, c;
-  sincos_reduced(a, s, c);
-  st(X, base, s);
-  for (int t = 0; t < nt; ++t) {
-    const int64_t k = base + (int64_t)(1 + t) * W;
-    st(X, k, c * Z[k]);
+  p.woff[p.H + 1] = p.woff[p.H] + in;  // output layer [1][w_{H-1}]
+  p.boff[p.H + 1] = p.boff[p.H] + 1;
+  p.n_params = p.woff[p.H + 1] + p.boff[p.H + 1];
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = (off + bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const int64_t R = p.rows;
+  const int H = p.H;
+  p.x4 = take((size_t)p.np * 16);
+  p.w0 = take((size_t)p.wp[0] * 16);
+  for (int j = 1; j < H; ++j) {
+    p.bh[j] = take((size_t)p.wp[j] * 4);
+    p.wf[j] = take((size_t)p.wp[j] * p.wp[j - 1] * p.esize);
+    p.wt[j] = take((size_t)p.wp[j] * p.wp[j - 1] * p.esize);
   }
-}
-
-// Backward through sin / the tangent products: Pbar -> Abar.
-template <class T>
-__global__ void k_train_act_bwd(const float* __restrict__ Z, const float* __restrict__ Pbar, int64_t n, int W,
-                                int nt, T* __restrict__ Abar, float* __restrict__ abar_val) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * W) return;
-  const int64_t p = i / W;
-  const int u = (int)(i % W);
-  const int64_t base = p * (1 + nt) * W + u;
-  float s, c;
-  sincos_reduced(Z[base], s, c);
-  float acc = c * Pbar[base];
-  for (int t = 0; t < nt; ++t) {
-    const int64_t k = base + (int64_t)(1 + t) * W;
-    const float pdb = Pbar[k];
-    acc = fmaf(-s * Z[k], pdb, acc);
-    st(Abar, k, c * pdb);
+  p.wl = take((size_t)p.wp[H - 1] * 4);
+  p.sg = take((size_t)p.np * 16);
+  int wmax = 0;
+  for (int k = 0; k < H - 1; ++k) {
+    p.X[k] = take((size_t)R * p.wp[k] * p.esize);
+    p.Xb[k] = take((size_t)R * p.wp[k] * p.esize);
+    if (k >= 1) p.Z[k] = take((size_t)R * p.wp[k] * 4);
   }
-  st(Abar, base, acc);
-  if (abar_val) abar_val[p * W + u] = acc;
-}
-
-// Output layer seeds: pbar = s * w_L (value row), pdbar_t = g_t * w_L.
-__global__ void k_train_seed(const float* __restrict__ s, const float* __restrict__ g, const float* __restrict__ wl,
-                             int64_t n, int W, int nt, float* __restrict__ Pbar) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * W) return;
-  const int64_t p = i / W;
-  const int u = (int)(i % W);
-  const int64_t base = p * (1 + nt) * W + u;
-  const float w = wl[u];
-  Pbar[base] = s ? s[p] * w : 0.f;
-  for (int t = 0; t < nt; ++t) Pbar[base + (int64_t)(1 + t) * W] = g ? g[p * nt + t] * w : 0.f;
-}
-
-// Four consecutive units per thread (W % 4 == 0): 16-byte loads of the FP32
-// blocks, 8- or 16-byte stores of the GEMM operands.  NT = tangent rows.
-template <class T> struct Vec4;
-template <> struct Vec4<float> { typedef float4 type; };
-template <> struct Vec4<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(4))); };
-template <> struct Vec4<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(4))); };
-
-template <class T>
-__device__ inline void st4(T* p, int64_t i, float4 v) {
-  typename Vec4<T>::type o;
-  o[0] = (T)v.x; o[1] = (T)v.y; o[2] = (T)v.z; o[3] = (T)v.w;
-  *reinterpret_cast<typename Vec4<T>::type*>(p + i) = o;
-}
-template <>
-__device__ inline void st4<float>(float* p, int64_t i, float4 v) { *reinterpret_cast<float4*>(p + i) = v; }
-__device__ inline float4 ld4(const float* p, int64_t i) { return *reinterpret_cast<const float4*>(p + i); }
-__device__ inline float4 ld4(const __bf16* p, int64_t i) {
-  const Vec4<__bf16>::type v = *reinterpret_cast<const Vec4<__bf16>::type*>(p + i);
-  return make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
-}
-__device__ inline float4 ld4(const _Float16* p, int64_t i) {
-  const Vec4<_Float16>::type v = *reinterpret_cast<const Vec4<_Float16>::type*>(p + i);
-  return make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
-}
-
-// K-blocked copy of a [n][4][W] operand block (three tangents), the layout of
-// the weight-gradient GEMM: element (row r = 4 p + t, unit u) at
-// ((r / 8) * W + u) * 8 + r % 8, so eight consecutive rows of one unit are 16
-// contiguous bytes.  vals[t][e]: rows t = 0..3 of point p, units u..u+3.
-template <class T>
-__device__ inline void st_blocked(T* Xb, int64_t p, int W, int u, const float (&vals)[4][4]) {
-  typedef T v4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v4 o;
-    o[0] = (T)vals[0][e]; o[1] = (T)vals[1][e]; o[2] = (T)vals[2][e]; o[3] = (T)vals[3][e];
-    *reinterpret_cast<v4*>(Xb + ((p >> 1) * W + u + e) * 8 + 4 * (p & 1)) = o;
+  for (int k = 1; k < H; ++k) wmax = p.wp[k] > wmax ? p.wp[k] : wmax;
+  for (int c = 0; c < 2; ++c) {
+    p.A[c] = take((size_t)R * wmax * p.esize);
+    p.Ab[c] = take((size_t)R * wmax * p.esize);
   }
-}
-
-// Two consecutive points (8 rows, point pair `pair`) at once: one 16-byte
-// store per unit.  lo / hi: rows of the even / odd point.
-template <class T>
-__device__ inline void st_blocked8(T* Xb, int64_t pair, int W, int u, const float (&lo)[4][4],
-                                   const float (&hi)[4][4]) {
-  typedef T v8 __attribute__((ext_vector_type(8)));
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v8 o;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      o[t] = (T)lo[t][e];
-      o[4 + t] = (T)hi[t][e];
-    }
-    *reinterpret_cast<v8*>(Xb + (pair * W + u + e) * 8) = o;
+  p.part_out = take((size_t)tgemm_part_floats(p.dtype, TG_OUT, R, p.wp[H - 1], 0) * 4);
+  int64_t pc = tgemm_part_floats(p.dtype, TG_OUT_BWD, R, p.wp[H - 1], 0);
+  for (int j = 2; j < H; ++j) {
+    const int64_t v = tgemm_part_floats(p.dtype, TG_ACT_BWD, R, p.wp[j - 1], 0);
+    pc = v > pc ? v : pc;
   }
-}
-
-// Blocked copy of P points' rows (P = 1: the point's half of each 16-byte
-// chunk; P = 2: whole chunks).
-template <class T, int P>
-__device__ inline void st_blocked_p(T* Xb, int64_t p_first, int W, int u, const float (&rows)[P][4][4]) {
-  if constexpr (P == 2) st_blocked8(Xb, p_first >> 1, W, u, rows[0], rows[1]);
-  else st_blocked(Xb, p_first, W, u, rows[0]);
-}
-
-template <class T, int NT, int P>
-__global__ __launch_bounds__(256) void k_train_first4(const float* __restrict__ x, const float* __restrict__ v,
-                                                      const float* __restrict__ W0, const float* __restrict__ b0,
-                                                      int64_t n, int W, float* __restrict__ Z, T* __restrict__ X,
-                                                      T* __restrict__ Xb) {
-  // P consecutive points x 4 units per thread
-  const int W4 = W / 4;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (n / P) * W4) return;
-  const int64_t pq = q / W4;
-  const int u = (int)(q - pq * W4) * 4;
-  float rows[P][4][4];
-#pragma unroll
-  for (int pp = 0; pp < P; ++pp) {
-    const int64_t p = pq * P + pp;
-    const int64_t base = p * (1 + NT) * W + u;
-    const float x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
-    float a[4], s[4], c[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float* w = W0 + 3 * (u + e);
-      a[e] = fmaf(w[2], x2, fmaf(w[1], x1, fmaf(w[0], x0, b0[u + e])));
-      sincos_reduced(a[e], s[e], c[e]);
-    }
-    *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
-    st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
-#pragma unroll
-    for (int e = 0; e < 4; ++e) rows[pp][0][e] = s[e];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const float* vt = v + (p * NT + t) * 3;
-      float ad[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float* w = W0 + 3 * (u + e);
-        ad[e] = fmaf(w[2], vt[2], fmaf(w[1], vt[1], w[0] * vt[0]));
-      }
-      const int64_t k = base + (int64_t)(1 + t) * W;
-      *reinterpret_cast<float4*>(Z + k) = make_float4(ad[0], ad[1], ad[2], ad[3]);
-      st4(X, k, make_float4(c[0] * ad[0], c[1] * ad[1], c[2] * ad[2], c[3] * ad[3]));
-      if constexpr (NT == 3) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = c[e] * ad[e];
-      }
-    }
+  const int64_t v0 = tgemm_part_floats(p.dtype, TG_FIRST_BWD, R, p.wp[0], 0);
+  p.part_col = take((size_t)(v0 > pc ? v0 : pc) * 4);
+  size_t dw = 0;
+  for (int j = 1; j < H; ++j) {
+    p.dw_chunks[j] = tgemm_weight_grad_chunks(p.dtype, R, p.wp[j], p.wp[j - 1]);
+    const size_t b = (size_t)tgemm_part_floats(p.dtype, TG_PART, p.wp[j], p.wp[j - 1], p.dw_chunks[j]) * 4;
+    dw = b > dw ? b : dw;
   }
-  if constexpr (NT == 3) {
-    if (Xb) st_blocked_p<T, P>(Xb, pq * P, W, u, rows);
-  }
-}
-
-template <class T, int NT, int P>
-__global__ __launch_bounds__(256) void k_train_act4(float* __restrict__ Z, const float* __restrict__ b, int64_t n,
-                                                    int W, T* __restrict__ X, T* __restrict__ Xb) {
-  // P consecutive points x 4 units per thread
-  const int W4 = W / 4;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (n / P) * W4) return;
-  const int64_t pq = q / W4;
-  const int u = (int)(q - pq * W4) * 4;
-  const float4 bb = *reinterpret_cast<const float4*>(b + u);
-  float rows[P][4][4];
-#pragma unroll
-  for (int pp = 0; pp < P; ++pp) {
-    const int64_t base = (pq * P + pp) * (1 + NT) * W + u;
-    const float4 z = ld4(Z, base);
-    const float a[4] = {z.x + bb.x, z.y + bb.y, z.z + bb.z, z.w + bb.w};
-    float s[4], c[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) sincos_reduced(a[e], s[e], c[e]);
-    *reinterpret_cast<float4*>(Z + base) = make_float4(a[0], a[1], a[2], a[3]);
-    st4(X, base, make_float4(s[0], s[1], s[2], s[3]));
-#pragma unroll
-    for (int e = 0; e < 4; ++e) rows[pp][0][e] = s[e];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int64_t k = base + (int64_t)(1 + t) * W;
-      const float4 ad = ld4(Z, k);
-      const float o[4] = {c[0] * ad.x, c[1] * ad.y, c[2] * ad.z, c[3] * ad.w};
-      st4(X, k, make_float4(o[0], o[1], o[2], o[3]));
-      if constexpr (NT == 3) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = o[e];
-      }
-    }
-  }
-  if constexpr (NT == 3) {
-    if (Xb) st_blocked_p<T, P>(Xb, pq * P, W, u, rows);
-  }
-}
-
-template <class T, int NT>
-__global__ __launch_bounds__(256) void k_train_act_bwd4(const float* __restrict__ Z, const float* __restrict__ Pbar,
-                                                        int64_t n, int W, T* __restrict__ Abar,
-                                                        float* __restrict__ abar_val, T* __restrict__ Ab) {
-  const int W4 = W / 4;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * W4) return;
-  const int64_t p = q / W4;
-  const int u = (int)(q - p * W4) * 4;
-  const int64_t base = p * (1 + NT) * W + u;
-  const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
-  const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
-  float s[4], c[4], acc[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    sincos_reduced(za[e], s[e], c[e]);
-    acc[e] = c[e] * pa[e];
-  }
-  float rows[4][4];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int64_t k = base + (int64_t)(1 + t) * W;
-    const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
-    const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] = fmaf(-s[e] * zv[e], pv[e], acc[e]);
-    const float o[4] = {c[0] * pv[0], c[1] * pv[1], c[2] * pv[2], c[3] * pv[3]};
-    st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
-    if constexpr (NT == 3) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) rows[1 + t][e] = o[e];
-    }
-  }
-  st4(Abar, base, make_float4(acc[0], acc[1], acc[2], acc[3]));
-  if (abar_val) *reinterpret_cast<float4*>(abar_val + p * W + u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
-  if constexpr (NT == 3) {
-    if (Ab) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) rows[0][e] = acc[e];
-      st_blocked(Ab, p, W, u, rows);
-    }
-  }
-}
-
-static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
-
-// Launch K<T, NT> for the run-time tangent count.
-#define MP_TRAIN_NT_DISPATCH(KERNEL, T, nt, grid, s, ...)                                     \
-  do {                                                                                        \
-    switch (nt) {                                                                             \
-      case 0: hipLaunchKernelGGL((KERNEL<T, 0>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
-      case 1: hipLaunchKernelGGL((KERNEL<T, 1>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
-      case 2: hipLaunchKernelGGL((KERNEL<T, 2>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((KERNEL<T, 3>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break; \
-    }                                                                                         \
-  } while (0)
-
-// The same for kernels with P points per thread.
-#define MP_TRAIN_NTP_DISPATCH(KERNEL, T, P, nt, grid, s, ...)                                       \
-  do {                                                                                              \
-    switch (nt) {                                                                                   \
-      case 0: hipLaunchKernelGGL((KERNEL<T, 0, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;   \
-      case 1: hipLaunchKernelGGL((KERNEL<T, 1, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;   \
-      case 2: hipLaunchKernelGGL((KERNEL<T, 2, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;   \
-      default: hipLaunchKernelGGL((KERNEL<T, 3, P>), dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;  \
-    }                                                                                               \
-  } while (0)
-
-}  // namespace mp
-
-using namespace mp;
-
-extern "C" {
-
-int mp_train_first(const float* x, const float* v, const float* W0, const float* b0, int64_t n, int W,
-                   int n_tangents, float* Z, void* X, int x_dtype, void* Xb, void* stream) {
-  MP_REQUIRE(x && W0 && b0 && Z && X && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_first: bad arguments");
-  MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first: tangent directions are NULL");
-  MP_REQUIRE(!Xb || (W % 4 == 0 && n_tangents == 3 && n % 2 == 0 && x_dtype != 0), MP_EINVAL,
-             "mp_train_first: the blocked copy needs 16-bit X, 3 tangents, even n and W % 4 == 0");
-  if (n == 0) return MP_OK;
-  hipStream_t s = as_stream(stream);
-  if (W % 4 == 0) {
-    // with the blocked copy: two points per thread (whole 16-byte chunks)
-    const unsigned g = grid_for((Xb ? n / 2 : n) * (W / 4));
-    if (x_dtype == 1 && Xb)
-      MP_TRAIN_NTP_DISPATCH(k_train_first4, __bf16, 2, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                            reinterpret_cast<__bf16*>(X), reinterpret_cast<__bf16*>(Xb));
-    else if (x_dtype == 1)
-      MP_TRAIN_NTP_DISPATCH(k_train_first4, __bf16, 1, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                            reinterpret_cast<__bf16*>(X), static_cast<__bf16*>(nullptr));
-    else if (x_dtype == 2 && Xb)
-      MP_TRAIN_NTP_DISPATCH(k_train_first4, _Float16, 2, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                            reinterpret_cast<_Float16*>(X), reinterpret_cast<_Float16*>(Xb));
-    else if (x_dtype == 2)
-      MP_TRAIN_NTP_DISPATCH(k_train_first4, _Float16, 1, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                            reinterpret_cast<_Float16*>(X), static_cast<_Float16*>(nullptr));
-    else
-      MP_TRAIN_NTP_DISPATCH(k_train_first4, float, 1, n_tangents, g, s, x, v, W0, b0, n, W, Z,
-                            reinterpret_cast<float*>(X), static_cast<float*>(nullptr));
-    MP_LAUNCHED("k_train_first4");
-    return MP_OK;
-  }
-  if (x_dtype == 1)
-    hipLaunchKernelGGL(k_train_first<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
-                       n_tangents, Z, reinterpret_cast<__bf16*>(X));
-  else if (x_dtype == 2)
-    hipLaunchKernelGGL(k_train_first<_Float16>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
-                       n_tangents, Z, reinterpret_cast<_Float16*>(X));
-  else
-    hipLaunchKernelGGL(k_train_first<float>, dim3(grid_for(n * W)), dim3(256), 0, s, x, v, W0, b0, n, W,
-                       n_tangents, Z, reinterpret_cast<float*>(X));
-  MP_LAUNCHED("k_train_first");
+  p.part_dw = take(dw);
+  p.tmp = take((size_t)TMP_FLOATS * 4);
+  p.total = off;
   return MP_OK;
 }
 
-int mp_train_act(float* Z, const float* b, int64_t n, int W, int n_tangents, void* X, int x_dtype, void* Xb,
-                 void* stream) {
-  MP_REQUIRE(Z && b && X && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_act: bad arguments");
-  MP_REQUIRE(!Xb || (W % 4 == 0 && n_tangents == 3 && n % 2 == 0 && x_dtype != 0), MP_EINVAL,
-             "mp_train_act: the blocked copy needs 16-bit X, 3 tangents, even n and W % 4 == 0");
-  if (n == 0) return MP_OK;
-  hipStream_t s = as_stream(stream);
-  if (W % 4 == 0) {
-    // with the blocked copy: two points per thread (whole 16-byte chunks)
-    const unsigned g = grid_for((Xb ? n / 2 : n) * (W / 4));
-    if (x_dtype == 1 && Xb)
-      MP_TRAIN_NTP_DISPATCH(k_train_act4, __bf16, 2, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X),
-                            reinterpret_cast<__bf16*>(Xb));
-    else if (x_dtype == 1)
-      MP_TRAIN_NTP_DISPATCH(k_train_act4, __bf16, 1, n_tangents, g, s, Z, b, n, W, reinterpret_cast<__bf16*>(X),
-                            static_cast<__bf16*>(nullptr));
-    else if (x_dtype == 2 && Xb)
-      MP_TRAIN_NTP_DISPATCH(k_train_act4, _Float16, 2, n_tangents, g, s, Z, b, n, W,
-                            reinterpret_cast<_Float16*>(X), reinterpret_cast<_Float16*>(Xb));
-    else if (x_dtype == 2)
-      MP_TRAIN_NTP_DISPATCH(k_train_act4, _Float16, 1, n_tangents, g, s, Z, b, n, W,
-                            reinterpret_cast<_Float16*>(X), static_cast<_Float16*>(nullptr));
-    else
-      MP_TRAIN_NTP_DISPATCH(k_train_act4, float, 1, n_tangents, g, s, Z, b, n, W, reinterpret_cast<float*>(X),
-                            static_cast<float*>(nullptr));
-    MP_LAUNCHED("k_train_act4");
-    return MP_OK;
+int ensure_train_ws(mp_stack* st, size_t bytes) {
+  if (st->train_ws_bytes >= bytes) return MP_OK;
+  if (st->train_ws) {
+    (void)hipFree(st->train_ws);
+    st->train_ws = nullptr;
+    st->train_ws_bytes = 0;
   }
-  if (x_dtype == 1)
-    hipLaunchKernelGGL(k_train_act<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
-                       reinterpret_cast<__bf16*>(X));
-  else if (x_dtype == 2)
-    hipLaunchKernelGGL(k_train_act<_Float16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
-                       reinterpret_cast<_Float16*>(X));
-  else
-    hipLaunchKernelGGL(k_train_act<float>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, b, n, W, n_tangents,
-                       reinterpret_cast<float*>(X));
-  MP_LAUNCHED("k_train_act");
-  return MP_OK;
-}
-
-int mp_train_act_bwd(const float* Z, const float* Pbar, int64_t n, int W, int n_tangents, void* Abar,
-                     int a_dtype, float* abar_val, void* Ab, void* stream) {
-  MP_REQUIRE(Z && Pbar && Abar && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_act_bwd: bad arguments");
-  MP_REQUIRE(!Ab || (W % 4 == 0 && n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
-             "mp_train_act_bwd: the blocked copy needs 16-bit A, 3 tangents, even n and W % 4 == 0");
-  if (n == 0) return MP_OK;
-  hipStream_t s = as_stream(stream);
-  if (W % 4 == 0) {
-    const unsigned g = grid_for(n * (W / 4));
-    if (a_dtype == 1)
-      MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, __bf16, n_tangents, g, s, Z, Pbar, n, W, reinterpret_cast<__bf16*>(Abar),
-                           abar_val, reinterpret_cast<__bf16*>(Ab));
-    else if (a_dtype == 2)
-      MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, _Float16, n_tangents, g, s, Z, Pbar, n, W,
-                           reinterpret_cast<_Float16*>(Abar), abar_val, reinterpret_cast<_Float16*>(Ab));
-    else
-      MP_TRAIN_NT_DISPATCH(k_train_act_bwd4, float, n_tangents, g, s, Z, Pbar, n, W, reinterpret_cast<float*>(Abar),
-                           abar_val, static_cast<float*>(nullptr));
-    MP_LAUNCHED("k_train_act_bwd4");
-    return MP_OK;
+  st->kept_level = -1;
+  const size_t b = (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+  hipError_t e = hipMalloc(&st->train_ws, b);
+  if (e != hipSuccess) {
+    set_error("training workspace allocation of %zu bytes failed: %s", b, hipGetErrorString(e));
+    return MP_ENOMEM;
   }
-  if (a_dtype == 1)
-    hipLaunchKernelGGL(k_train_act_bwd<__bf16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W, n_tangents,
-                       reinterpret_cast<__bf16*>(Abar), abar_val);
-  else if (a_dtype == 2)
-    hipLaunchKernelGGL(k_train_act_bwd<_Float16>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W,
-                       n_tangents, reinterpret_cast<_Float16*>(Abar), abar_val);
-  else
-    hipLaunchKernelGGL(k_train_act_bwd<float>, dim3(grid_for(n * W)), dim3(256), 0, s, Z, Pbar, n, W, n_tangents,
-                       reinterpret_cast<float*>(Abar), abar_val);
-  MP_LAUNCHED("k_train_act_bwd");
+  st->train_ws_bytes = b;
   return MP_OK;
 }
 
-int mp_train_seed(const float* s_val, const float* g_tan, const float* wl, int64_t n, int W, int n_tangents,
-                  float* Pbar, void* stream) {
-  MP_REQUIRE(wl && Pbar && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_seed: bad arguments");
-  if (n == 0) return MP_OK;
-  hipLaunchKernelGGL(k_train_seed, dim3(grid_for(n * W)), dim3(256), 0, as_stream(stream), s_val, g_tan, wl, n, W,
-                     n_tangents, Pbar);
-  MP_LAUNCHED("k_train_seed");
-  return MP_OK;
+inline unsigned blocks_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// dst[i] = i < n ? src[i] : 0 for i < np
+__global__ void k_tw_pad(const float* __restrict__ src, int64_t n, float* __restrict__ dst, int64_t np) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) dst[i] = i < n ? src[i] : 0.f;
 }
 
-}  // extern "C"
+// x4[p] = (x, y, z, 0) of point p (zero past n)
+__global__ void k_tw_points(const float* __restrict__ x, int64_t n, float4* __restrict__ x4, int64_t np) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) x4[p] = p < n ? make_float4(x[3 * p], x[3 * p + 1], x[3 * p + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
 
-// ---------------------------------------------------------------------------
-// Output layer (W -> 1) of the training forward / backward, fused:
-//   forward:  f = w_L . sin(a) + b_L,  fd_t = w_L . (cos(a) ad_t)
-//   backward: Pbar = [s w_L; g_t w_L] and the partial sums of
-//             dw_L = sum_points s sin(a) + sum_t g_t cos(a) ad_t
-// One wave per point for the forward (lane-strided dot products, then a
-// wave reduction); the backward accumulates dw_L per block.
-// ---------------------------------------------------------------------------
-namespace mp {
+// w0[u] = (W_0[u][0..2], b_0[u]) (zero past w)
+__global__ void k_tw_first_w(const float* __restrict__ W0, const float* __restrict__ b0, int w, float4* __restrict__ w0,
+                             int wp) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < wp) w0[u] = u < w ? make_float4(W0[3 * u], W0[3 * u + 1], W0[3 * u + 2], b0[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
 
-// b_sine (may be NULL): bias of the last sine layer, added to the value row
-// of Z in place first (Z then holds that layer's full pre-activations).
-__global__ __launch_bounds__(256) void k_train_out_fwd(float* __restrict__ Z, const float* __restrict__ b_sine,
-                                                       const float* __restrict__ wl, const float* __restrict__ bl,
-                                                       int64_t n, int W, int nt, float* __restrict__ f,
-                                                       float* __restrict__ fd) {
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+// sg[p] = (dL/df, dL/dgrad f) of point p (NULL seeds and points past n: zero)
+__global__ void k_tw_seeds(const float* __restrict__ s, const float* __restrict__ g, int64_t n, float4* __restrict__ sg,
+                           int64_t np) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (p < n) {
+    if (s) v.x = s[p];
+    if (g) {
+      v.y = g[3 * p];
+      v.z = g[3 * p + 1];
+      v.w = g[3 * p + 2];
+    }
+  }
+  sg[p] = v;
+}
+
+// hidden layer j as the B operands of the training GEMMs, zero padded, in
+// the element type and the blocked layout [K/KL][N][KL]: Wf (forward: N = out,
+// K = in) and Wt (backward: N = in, K = out)
+template <class T>
+__global__ void k_tw_hidden(const float* __restrict__ W, int wo, int wi, int wpo, int wpi, T* __restrict__ Wf,
+                            T* __restrict__ Wt) {
+  constexpr int KL = 16 / (int)sizeof(T);
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)wpo * wpi) return;
+  const int i = (int)(e / wpi), k = (int)(e % wpi);
+  const float v = (i < wo && k < wi) ? W[(int64_t)i * wi + k] : 0.f;
+  Wf[((int64_t)(k / KL) * wpo + i) * KL + k % KL] = (T)v;
+  Wt[((int64_t)(i / KL) * wpi + k) * KL + i % KL] = (T)v;
+}
+
+// Layer 0: a = W_0 x + b_0 and the rows (sin a, cos(a) W_0[u][0..2]) into X
+// (unit-blocked [W/KL][4 np][KL]) and Xb (row-blocked [4 np / KL][W][KL]).
+// One wave per 16 points x 64 units: lane u computes its unit's rows for the
+// 16 points into LDS, then every store instruction writes 1 KB of contiguous
+// memory: X by rows (lane l = row l of the 64 rows, one 16-byte chunk of KL
+// units), Xb by units (lane l = unit l, one row group).  The grid is exact
+// (np / 16 x W / 64 waves, a multiple of four).
+template <class T>
+__global__ __launch_bounds__(256) void k_tw_first(const float4* __restrict__ x4, const float4* __restrict__ w0,
+                                                  int64_t np, int W, T* __restrict__ X, T* __restrict__ Xb) {
+  constexpr int KL = 16 / (int)sizeof(T);
+  constexpr int PW = 16;  // points per wave
+  typedef T Vec __attribute__((ext_vector_type(KL)));
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  __shared__ T lds[4][PW][65][4];  // [wave][point][unit (+1 against bank conflicts)][row]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
+  const int ublocks = W / 64;
+  const int64_t p0 = (wid / ublocks) * PW;
+  const int u0 = (int)(wid % ublocks) * 64;
+  const int64_t M = 4 * np;
+  const float4 w = w0[u0 + lane];
+#pragma unroll 4
+  for (int pt = 0; pt < PW; ++pt) {
+    const float4 xp = x4[p0 + pt];
+    float sn, cs;
+    sincos_reduced(fmaf(w.z, xp.z, fmaf(w.y, xp.y, fmaf(w.x, xp.x, w.w))), sn, cs);
+    V4 o;
+    o[0] = (T)sn;
+    o[1] = (T)(cs * w.x);
+    o[2] = (T)(cs * w.y);
+    o[3] = (T)(cs * w.z);
+    *reinterpret_cast<V4*>(&lds[wave][pt][lane][0]) = o;
+  }
+  __syncthreads();
+  // X: chunk c holds units u0 + c KL .. + KL - 1; lane = row 4 (lane / 4) + lane % 4
+#pragma unroll
+  for (int c = 0; c < 64 / KL; ++c) {
+    Vec o;
+#pragma unroll
+    for (int k = 0; k < KL; ++k) o[k] = lds[wave][lane / 4][c * KL + k][lane % 4];
+    *reinterpret_cast<Vec*>(X + ((int64_t)(u0 / KL + c) * M + 4 * p0 + lane) * KL) = o;
+  }
+  // Xb: row groups of KL rows (two points in 16 bit, one in FP32); lane = unit
+#pragma unroll
+  for (int rg = 0; rg < 4 * PW / KL; ++rg) {
+    Vec o;
+#pragma unroll
+    for (int k = 0; k < KL; ++k) o[k] = lds[wave][(rg * KL + k) / 4][lane][k % 4];
+    *reinterpret_cast<Vec*>(Xb + ((4 * p0 / KL + rg) * W + u0 + lane) * KL) = o;
+  }
+}
+
+// f[p] = b_out + sum of the column-range partials, grad[p] likewise (p < n)
+__global__ void k_tw_out_sum(const float4* __restrict__ part, int slots, int64_t np, const float* __restrict__ b_out,
+                             int64_t n, float* __restrict__ f, float* __restrict__ grad) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  float* z = Z + p * (1 + nt) * W;
-  for (int u = lane; u < W; u += 64) {
-    float a = z[u];
-    if (b_sine) {
-      a += b_sine[u];
-      z[u] = a;
-    }
-    float s, c;
-    sincos_reduced(a, s, c);
-    const float w = wl[u];
-    acc[0] = fmaf(w, s, acc[0]);
-    for (int t = 0; t < nt; ++t) acc[1 + t] = fmaf(w * c, z[(1 + t) * W + u], acc[1 + t]);
+  float4 acc = part[p];
+  for (int k = 1; k < slots; ++k) {
+    const float4 v = part[(int64_t)k * np + p];
+    acc.x += v.x;
+    acc.y += v.y;
+    acc.z += v.z;
+    acc.w += v.w;
   }
-  for (int k = 0; k <= nt; ++k)
-    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
-  if (lane == 0) {
-    f[p] = acc[0] + bl[0];
-    for (int t = 0; t < nt; ++t) fd[p * nt + t] = acc[1 + t];
+  if (f) f[p] = acc.x + b_out[0];
+  if (grad) {
+    grad[3 * p] = acc.y;
+    grad[3 * p + 1] = acc.z;
+    grad[3 * p + 2] = acc.w;
   }
 }
 
-// The same with 16 lanes per point (W % 64 == 0): each lane reads 4 units
-// per row with one 16-byte load, a row's 16 lanes 256 contiguous bytes, and
-// the sums reduce over 16 lanes; 16 points per block.
-template <int NT>
-__global__ __launch_bounds__(256) void k_train_out_fwd16(float* __restrict__ Z, const float* __restrict__ b_sine,
-                                                         const float* __restrict__ wl, const float* __restrict__ bl,
-                                                         int64_t n, int W, float* __restrict__ f,
-                                                         float* __restrict__ fd) {
-  const int64_t p = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int j = threadIdx.x & 15;
-  if (p >= n) return;  // the whole 16-lane group
-  float acc[1 + NT];
-#pragma unroll
-  for (int k = 0; k <= NT; ++k) acc[k] = 0.f;
-  float* z = Z + p * (1 + NT) * W;
-  for (int u = 4 * j; u < W; u += 64) {
-    float4 a = ld4(z, u);
-    if (b_sine) {
-      const float4 bb = ld4(b_sine, u);
-      a = make_float4(a.x + bb.x, a.y + bb.y, a.z + bb.z, a.w + bb.w);
-      *reinterpret_cast<float4*>(z + u) = a;
-    }
-    const float4 w4 = ld4(wl, u);
-    const float av[4] = {a.x, a.y, a.z, a.w}, wv[4] = {w4.x, w4.y, w4.z, w4.w};
-    float sn[4], cs[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      sincos_reduced(av[e], sn[e], cs[e]);
-      acc[0] = fmaf(wv[e], sn[e], acc[0]);
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const float4 zt = ld4(z, (int64_t)(1 + t) * W + u);
-      const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc[1 + t] = fmaf(wv[e] * cs[e], zv[e], acc[1 + t]);
-    }
+// out[0] = sum_i src[i * stride] for i < n (one workgroup)
+__global__ __launch_bounds__(1024) void k_tw_sum(const float* __restrict__ src, int64_t n, int stride,
+                                                 float* __restrict__ out) {
+  __shared__ float red[1024];
+  float acc = 0.f;
+#pragma unroll 8
+  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += src[i * stride];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
   }
-#pragma unroll
-  for (int k = 0; k <= NT; ++k)
-    for (int o = 8; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
-  if (j == 0) {
-    f[p] = acc[0] + bl[0];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) fd[p * NT + t] = acc[1 + t];
-  }
+  if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// grid.x = ceil(n / rows_per_block), block = 256 threads over W (W <= 512:
-// two units per thread); dwl_part [gridDim.x][W].
-__global__ __launch_bounds__(256) void k_train_out_bwd(const float* __restrict__ Z, const float* __restrict__ s,
-                                                       const float* __restrict__ g, const float* __restrict__ wl,
-                                                       int64_t n, int W, int nt, int rows_per_block,
-                                                       float* __restrict__ Pbar, float* __restrict__ dwl_part) {
-  const int64_t p0 = (int64_t)blockIdx.x * rows_per_block;
-  for (int u = threadIdx.x; u < W; u += blockDim.x) {
-    const float w = wl[u];
-    float acc = 0.f;
-    for (int r = 0; r < rows_per_block; ++r) {
-      const int64_t p = p0 + r;
-      if (p >= n) break;
-      const float* z = Z + p * (1 + nt) * W;
-      float* pb = Pbar + p * (1 + nt) * W;
-      float sn, cs;
-      sincos_reduced(z[u], sn, cs);
-      const float sv = s ? s[p] : 0.f;
-      pb[u] = sv * w;
-      acc = fmaf(sv, sn, acc);
-      for (int t = 0; t < nt; ++t) {
-        const float gv = g ? g[p * nt + t] : 0.f;
-        pb[(1 + t) * W + u] = gv * w;
-        acc = fmaf(gv * cs, z[(1 + t) * W + u], acc);
-      }
-    }
-    dwl_part[(int64_t)blockIdx.x * W + u] = acc;
-  }
+// tmp[slice][c] = sum over parts p = slice, slice + S, ... of part[p][c]
+__global__ void k_sum_slices(const float* __restrict__ part, int64_t P, int64_t C, int S, float* __restrict__ tmp) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int slice = blockIdx.y;
+  if (c >= C) return;
+  float acc = 0.f;
+#pragma unroll 8
+  for (int64_t p = slice; p < P; p += S) acc += part[p * C + c];
+  tmp[(int64_t)slice * C + c] = acc;
 }
 
-// Rows (points) per block of the fused reductions below.
-constexpr int RED_ROWS = 64;
-
-// Output layer backward, four units per thread (W % 4 == 0, blockDim = W / 4).
-template <int NT>
-__global__ __launch_bounds__(128) void k_train_out_bwd4(const float* __restrict__ Z, const float* __restrict__ s,
-                                                        const float* __restrict__ g, const float* __restrict__ wl,
-                                                        int64_t n, int W, float* __restrict__ Pbar,
-                                                        float* __restrict__ dwl_part) {
-  const int u = threadIdx.x * 4;
-  if (u >= W) return;
-  const float4 w4 = ld4(wl, u);
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
-  for (int r = 0; r < RED_ROWS; ++r) {
-    const int64_t p = p0 + r;
-    if (p >= n) break;
-    const int64_t base = p * (1 + NT) * W + u;
-    const float4 z = ld4(Z, base);
-    const float za[4] = {z.x, z.y, z.z, z.w};
-    float sn[4], cs[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) sincos_reduced(za[e], sn[e], cs[e]);
-    const float sv = s ? s[p] : 0.f;
-    *reinterpret_cast<float4*>(Pbar + base) = make_float4(sv * w4.x, sv * w4.y, sv * w4.z, sv * w4.w);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] = fmaf(sv, sn[e], acc[e]);
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int64_t k = base + (int64_t)(1 + t) * W;
-      const float gv = g ? g[p * NT + t] : 0.f;
-      *reinterpret_cast<float4*>(Pbar + k) = make_float4(gv * w4.x, gv * w4.y, gv * w4.z, gv * w4.w);
-      const float4 zt = ld4(Z, k);
-      const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] = fmaf(gv * cs[e], zv[e], acc[e]);
-    }
-  }
-  *reinterpret_cast<float4*>(dwl_part + (int64_t)blockIdx.x * W + u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+// out[r * out_rs + c] = sum_s tmp[s][off + r * rs + c * cs], r < rows, c < cols
+__global__ void k_gather_sum(const float* __restrict__ tmp, int S, int64_t C, int rows, int cols, int64_t rs,
+                             int64_t cs, int64_t off, float* __restrict__ out, int64_t out_rs) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)rows * cols) return;
+  const int r = (int)(e / cols), c = (int)(e % cols);
+  const int64_t src = off + r * rs + c * cs;
+  float acc = 0.f;
+  for (int sl = 0; sl < S; ++sl) acc += tmp[(int64_t)sl * C + src];
+  out[(int64_t)r * out_rs + c] = acc;
 }
 
-// Output layer and last sine layer backward in one pass (W % 4 == 0,
-// blockDim = W / 4).  The seeds of the last sine layer are rank one,
-// pbar = s w_L and pdbar_t = g_t w_L, so they are formed in registers:
-//   abar = cos a pbar - sin a sum_t ad_t pdbar_t,  adbar_t = cos a pdbar_t
-// are written as the GEMM operand (and its K-blocked copy), and
-// part [blocks][2][W] receives per-block partial sums of
-// dw_L = sum s sin a + sum_t g_t cos a ad_t (row 0) and db = sum abar (row 1).
-template <class T, int NT, int P>
-__global__ __launch_bounds__(128) void k_train_out_act_bwd4(const float* __restrict__ Z, const float* __restrict__ s,
-                                                            const float* __restrict__ g, const float* __restrict__ wl,
-                                                            int64_t n, int W, T* __restrict__ Abar, T* __restrict__ Ab,
-                                                            float* __restrict__ part) {
-  const int u = threadIdx.x * 4;
-  if (u >= W) return;
-  const float4 w4 = ld4(wl, u);
-  const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
-  float dwl[4] = {0.f, 0.f, 0.f, 0.f}, db[4] = {0.f, 0.f, 0.f, 0.f};
-  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
-  for (int r = 0; r < RED_ROWS; r += P) {
-    if (p0 + r >= n) break;  // n is even when P = 2
-    float rows[P][4][4];
-#pragma unroll
-    for (int pp = 0; pp < P; ++pp) {
-      const int64_t p = p0 + r + pp;
-      const int64_t base = p * (1 + NT) * W + u;
-      const float4 z = ld4(Z, base);
-      const float za[4] = {z.x, z.y, z.z, z.w};
-      float sn[4], cs[4], abar[4];
-      const float sv = s ? s[p] : 0.f;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        sincos_reduced(za[e], sn[e], cs[e]);
-        abar[e] = cs[e] * (sv * wv[e]);
-        dwl[e] = fmaf(sv, sn[e], dwl[e]);
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int64_t k = base + (int64_t)(1 + t) * W;
-        const float gv = g ? g[p * NT + t] : 0.f;
-        const float4 zt = ld4(Z, k);
-        const float zv[4] = {zt.x, zt.y, zt.z, zt.w};
-        float o[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const float pt = gv * wv[e];
-          abar[e] = fmaf(-sn[e] * zv[e], pt, abar[e]);
-          o[e] = cs[e] * pt;
-          dwl[e] = fmaf(gv * cs[e], zv[e], dwl[e]);
-        }
-        st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
-        if constexpr (NT == 3) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = o[e];
-        }
-      }
-      st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
-#pragma unroll
-      for (int e = 0; e < 4; ++e) db[e] += abar[e];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) rows[pp][0][e] = abar[e];
-    }
-    if constexpr (NT == 3) {
-      if (Ab) st_blocked_p<T, P>(Ab, p0 + r, W, u, rows);
-    }
-  }
-  float* o = part + (int64_t)blockIdx.x * 2 * W + u;
-  *reinterpret_cast<float4*>(o) = make_float4(dwl[0], dwl[1], dwl[2], dwl[3]);
-  *reinterpret_cast<float4*>(o + W) = make_float4(db[0], db[1], db[2], db[3]);
-}
-
-// Hidden-layer backward through sin / the tangent products (as
-// k_train_act_bwd4) with the bias gradient reduced per block: blockDim =
-// W / 4, db_part [blocks][W].  P: element type of Pbar.
-template <class T, class PB, int NT, int P>
-__global__ __launch_bounds__(128) void k_train_act_bwd_part4(const float* __restrict__ Z,
-                                                             const PB* __restrict__ Pbar, int64_t n, int W,
-                                                             T* __restrict__ Abar, T* __restrict__ Ab,
-                                                             float* __restrict__ db_part) {
-  const int u = threadIdx.x * 4;
-  if (u >= W) return;
-  float db[4] = {0.f, 0.f, 0.f, 0.f};
-  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
-  for (int r = 0; r < RED_ROWS; r += P) {
-    if (p0 + r >= n) break;  // n is even when P = 2
-    float rows[P][4][4];
-#pragma unroll
-    for (int pp = 0; pp < P; ++pp) {
-      const int64_t p = p0 + r + pp;
-      const int64_t base = p * (1 + NT) * W + u;
-      const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
-      const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
-      float sn[4], cs[4], abar[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        sincos_reduced(za[e], sn[e], cs[e]);
-        abar[e] = cs[e] * pa[e];
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int64_t k = base + (int64_t)(1 + t) * W;
-        const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
-        const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
-        float o[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          abar[e] = fmaf(-sn[e] * zv[e], pv[e], abar[e]);
-          o[e] = cs[e] * pv[e];
-        }
-        st4(Abar, k, make_float4(o[0], o[1], o[2], o[3]));
-        if constexpr (NT == 3) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) rows[pp][1 + t][e] = o[e];
-        }
-      }
-      st4(Abar, base, make_float4(abar[0], abar[1], abar[2], abar[3]));
-#pragma unroll
-      for (int e = 0; e < 4; ++e) db[e] += abar[e];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) rows[pp][0][e] = abar[e];
-    }
-    if constexpr (NT == 3) {
-      if (Ab) st_blocked_p<T, P>(Ab, p0 + r, W, u, rows);
-    }
-  }
-  *reinterpret_cast<float4*>(db_part + (int64_t)blockIdx.x * W + u) = make_float4(db[0], db[1], db[2], db[3]);
-}
-
-// Layer-0 backward fused with its parameter gradients (W % 4 == 0,
-// blockDim = W / 4): abar = cos a pbar - sin a sum_t ad_t pdbar_t and
-// adbar_t = cos a pdbar_t are reduced in place against the layer input
-// [x; v_t], giving per-block partial sums part [blocks][W][4] of
-// (dW0[u][0..2], db0[u]).
-template <class P, int NT>
-__global__ __launch_bounds__(128) void k_train_first_bwd4(const float* __restrict__ Z, const P* __restrict__ Pbar,
-                                                          const float* __restrict__ x, const float* __restrict__ v,
-                                                          int64_t n, int W, float* __restrict__ part) {
-  const int u = threadIdx.x * 4;
-  if (u >= W) return;
-  float acc[4][4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc[e][k] = 0.f;
-  const int64_t p0 = (int64_t)blockIdx.x * RED_ROWS;
-  for (int r = 0; r < RED_ROWS; ++r) {
-    const int64_t p = p0 + r;
-    if (p >= n) break;
-    const int64_t base = p * (1 + NT) * W + u;
-    const float4 z = ld4(Z, base), pb = ld4(Pbar, base);
-    const float za[4] = {z.x, z.y, z.z, z.w}, pa[4] = {pb.x, pb.y, pb.z, pb.w};
-    float sn[4], cs[4], abar[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      sincos_reduced(za[e], sn[e], cs[e]);
-      abar[e] = cs[e] * pa[e];
-    }
-    const float x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int64_t k = base + (int64_t)(1 + t) * W;
-      const float4 zt = ld4(Z, k), pt = ld4(Pbar, k);
-      const float zv[4] = {zt.x, zt.y, zt.z, zt.w}, pv[4] = {pt.x, pt.y, pt.z, pt.w};
-      const float* vt = v + (p * NT + t) * 3;
-      const float v0 = vt[0], v1 = vt[1], v2 = vt[2];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        abar[e] = fmaf(-sn[e] * zv[e], pv[e], abar[e]);
-        const float ad = cs[e] * pv[e];
-        acc[e][0] = fmaf(ad, v0, acc[e][0]);
-        acc[e][1] = fmaf(ad, v1, acc[e][1]);
-        acc[e][2] = fmaf(ad, v2, acc[e][2]);
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      acc[e][0] = fmaf(abar[e], x0, acc[e][0]);
-      acc[e][1] = fmaf(abar[e], x1, acc[e][1]);
-      acc[e][2] = fmaf(abar[e], x2, acc[e][2]);
-      acc[e][3] += abar[e];
-    }
-  }
-  float* o = part + ((int64_t)blockIdx.x * W + u) * 4;
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    *reinterpret_cast<float4*>(o + 4 * e) = make_float4(acc[e][0], acc[e][1], acc[e][2], acc[e][3]);
-}
-
-}  // namespace mp
-
-extern "C" {
-
-int mp_train_out_fwd(float* Z, const float* b_sine, const float* w_last, const float* b_last, int64_t n, int W,
-                     int n_tangents, float* f, float* fd, void* stream) {
-  MP_REQUIRE(Z && w_last && b_last && f && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_out_fwd: bad arguments");
-  MP_REQUIRE(n_tangents == 0 || fd, MP_EINVAL, "mp_train_out_fwd: fd is NULL");
-  if (n == 0) return MP_OK;
-  hipStream_t st = as_stream(stream);
-  if (W % 64 == 0) {
-    const dim3 g((unsigned)((n + 15) / 16));
-    switch (n_tangents) {
-      case 0: hipLaunchKernelGGL(k_train_out_fwd16<0>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
-      case 1: hipLaunchKernelGGL(k_train_out_fwd16<1>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
-      case 2: hipLaunchKernelGGL(k_train_out_fwd16<2>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
-      default: hipLaunchKernelGGL(k_train_out_fwd16<3>, g, dim3(256), 0, st, Z, b_sine, w_last, b_last, n, W, f, fd); break;
-    }
-    MP_LAUNCHED("k_train_out_fwd16");
-    return MP_OK;
-  }
-  hipLaunchKernelGGL(k_train_out_fwd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, Z, b_sine,
-                     w_last, b_last, n, W, n_tangents, f, fd);
-  MP_LAUNCHED("k_train_out_fwd");
+// Sum P parts of C floats each, then write a rows x cols window of the sum.
+int reduce_parts(const float* part, int64_t P, int64_t C, float* tmp, int rows, int cols, int64_t rs, int64_t cs,
+                 int64_t off, float* out, int64_t out_rs, hipStream_t s) {
+  int S = (int)(P / 64);
+  S = S < 1 ? 1 : S > 64 ? 64 : S;
+  MP_REQUIRE((int64_t)S * C <= TMP_FLOATS, MP_EINVAL, "training: reduction of %lld x %lld", (long long)P,
+             (long long)C);
+  hipLaunchKernelGGL(k_sum_slices, dim3(blocks_of(C, 256), S), dim3(256), 0, s, part, P, C, S, tmp);
+  MP_LAUNCHED("k_sum_slices");
+  hipLaunchKernelGGL(k_gather_sum, dim3(blocks_of((int64_t)rows * cols, 256)), dim3(256), 0, s, tmp, S, C, rows,
+                     cols, rs, cs, off, out, out_rs);
+  MP_LAUNCHED("k_gather_sum");
   return MP_OK;
 }
 
-int64_t mp_train_out_bwd_blocks(int64_t n) { return (n + RED_ROWS - 1) / RED_ROWS; }
+inline char* at(const mp_stack* st, size_t off) { return static_cast<char*>(st->train_ws) + off; }
+inline float* atf(const mp_stack* st, size_t off) { return reinterpret_cast<float*>(at(st, off)); }
+inline float4* at4(const mp_stack* st, size_t off) { return reinterpret_cast<float4*>(at(st, off)); }
 
-int mp_train_out_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
-                     int n_tangents, float* Pbar, float* dwl_part, void* stream) {
-  MP_REQUIRE(Z && w_last && Pbar && dwl_part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_out_bwd: bad arguments");
-  if (n == 0) return MP_OK;
-  const int64_t blocks = mp_train_out_bwd_blocks(n);
-  hipStream_t st = as_stream(stream);
-  if (W % 4 == 0 && W <= 512) {
-    switch (n_tangents) {
-      case 0: hipLaunchKernelGGL(k_train_out_bwd4<0>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
-      case 1: hipLaunchKernelGGL(k_train_out_bwd4<1>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
-      case 2: hipLaunchKernelGGL(k_train_out_bwd4<2>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
-      default: hipLaunchKernelGGL(k_train_out_bwd4<3>, dim3((unsigned)blocks), dim3(W / 4), 0, st, Z, s, g, w_last, n, W, Pbar, dwl_part); break;
-    }
-    MP_LAUNCHED("k_train_out_bwd4");
-    return MP_OK;
+// padded operands from the level's FP32 weights (device) and the points
+int stage_inputs(const mp_stack* st, const mp_stack::Level& L, const TrainPlan& p, const float* x, hipStream_t s) {
+  const int H = p.H;
+  hipLaunchKernelGGL(k_tw_points, dim3(blocks_of(p.np, 256)), dim3(256), 0, s, x, p.n, at4(st, p.x4), p.np);
+  hipLaunchKernelGGL(k_tw_first_w, dim3(blocks_of(p.wp[0], 256)), dim3(256), 0, s, L.weights, L.biases, p.w[0],
+                     at4(st, p.w0), p.wp[0]);
+  for (int j = 1; j < H; ++j) {
+    hipLaunchKernelGGL(k_tw_pad, dim3(blocks_of(p.wp[j], 256)), dim3(256), 0, s, L.biases + p.boff[j],
+                       (int64_t)p.w[j], atf(st, p.bh[j]), (int64_t)p.wp[j]);
+    const unsigned gb = blocks_of((int64_t)p.wp[j] * p.wp[j - 1], 256);
+    const float* Wj = L.weights + p.woff[j];
+    if (p.dtype == 1)
+      hipLaunchKernelGGL(k_tw_hidden<__bf16>, dim3(gb), dim3(256), 0, s, Wj, p.w[j], p.w[j - 1], p.wp[j], p.wp[j - 1],
+                         reinterpret_cast<__bf16*>(at(st, p.wf[j])), reinterpret_cast<__bf16*>(at(st, p.wt[j])));
+    else if (p.dtype == 2)
+      hipLaunchKernelGGL(k_tw_hidden<_Float16>, dim3(gb), dim3(256), 0, s, Wj, p.w[j], p.w[j - 1], p.wp[j],
+                         p.wp[j - 1], reinterpret_cast<_Float16*>(at(st, p.wf[j])),
+                         reinterpret_cast<_Float16*>(at(st, p.wt[j])));
+    else
+      hipLaunchKernelGGL(k_tw_hidden<float>, dim3(gb), dim3(256), 0, s, Wj, p.w[j], p.w[j - 1], p.wp[j], p.wp[j - 1],
+                         atf(st, p.wf[j]), atf(st, p.wt[j]));
   }
-  hipLaunchKernelGGL(k_train_out_bwd, dim3((unsigned)blocks), dim3(256), 0, st, Z, s, g, w_last, n,
-                     W, n_tangents, RED_ROWS, Pbar, dwl_part);
-  MP_LAUNCHED("k_train_out_bwd");
+  hipLaunchKernelGGL(k_tw_pad, dim3(blocks_of(p.wp[H - 1], 256)), dim3(256), 0, s, L.weights + p.woff[H],
+                     (int64_t)p.w[H - 1], atf(st, p.wl), (int64_t)p.wp[H - 1]);
+  MP_LAUNCHED("training inputs");
   return MP_OK;
 }
 
-// Launch K<T, NT, P> over `blocks` blocks of W / 4 threads.
-#define MP_TRAIN_REDP_DISPATCH(KERNEL, T, P, nt, blocks, W, s, ...)                                                  \
-  do {                                                                                                              \
-    switch (nt) {                                                                                                   \
-      case 0: hipLaunchKernelGGL((KERNEL<T, 0, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-      case 1: hipLaunchKernelGGL((KERNEL<T, 1, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-      case 2: hipLaunchKernelGGL((KERNEL<T, 2, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((KERNEL<T, 3, P>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-    }                                                                                                               \
-  } while (0)
+TgArgs gemm_args(const void* A, const void* B, int64_t M, int N, int K) {
+  TgArgs a{};
+  a.A = A;
+  a.B = B;
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  return a;
+}
 
-// Launch K<T, NT> over `blocks` blocks of W / 4 threads.
-#define MP_TRAIN_RED_DISPATCH(KERNEL, T, nt, blocks, W, s, ...)                                              \
-  do {                                                                                                      \
-    switch (nt) {                                                                                           \
-      case 0: hipLaunchKernelGGL((KERNEL<T, 0>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-      case 1: hipLaunchKernelGGL((KERNEL<T, 1>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-      case 2: hipLaunchKernelGGL((KERNEL<T, 2>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((KERNEL<T, 3>), dim3((unsigned)(blocks)), dim3((W) / 4), 0, s, __VA_ARGS__); break; \
-    }                                                                                                       \
-  } while (0)
-
-int mp_train_out_act_bwd(const float* Z, const float* s, const float* g, const float* w_last, int64_t n, int W,
-                         int n_tangents, void* Abar, int a_dtype, void* Ab, float* part, void* stream) {
-  MP_REQUIRE(Z && w_last && Abar && part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_out_act_bwd: bad arguments");
-  MP_REQUIRE(a_dtype >= 0 && a_dtype <= 2, MP_EINVAL, "mp_train_out_act_bwd: a_dtype %d", a_dtype);
-  MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED, "mp_train_out_act_bwd: width %d (needs a multiple of 4, <= 512)",
-             W);
-  MP_REQUIRE(!Ab || (n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
-             "mp_train_out_act_bwd: the blocked copy needs 16-bit A, 3 tangents and even n");
-  if (n == 0) return MP_OK;
-  const int64_t blocks = mp_train_out_bwd_blocks(n);
-  hipStream_t st = as_stream(stream);
-  // with the blocked copy: two points per step (whole 16-byte chunks)
-  if (a_dtype == 1 && Ab)
-    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, __bf16, 2, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                           reinterpret_cast<__bf16*>(Abar), reinterpret_cast<__bf16*>(Ab), part);
-  else if (a_dtype == 1)
-    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, __bf16, 1, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                           reinterpret_cast<__bf16*>(Abar), static_cast<__bf16*>(nullptr), part);
-  else if (a_dtype == 2 && Ab)
-    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, _Float16, 2, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                           reinterpret_cast<_Float16*>(Abar), reinterpret_cast<_Float16*>(Ab), part);
-  else if (a_dtype == 2)
-    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, _Float16, 1, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                           reinterpret_cast<_Float16*>(Abar), static_cast<_Float16*>(nullptr), part);
+// the forward with tangents: X_k, Xb_k (k <= H-2), Z_k (1 <= k <= H-2); with
+// f or grad, also the output (layer H-1 and the output layer)
+int run_forward(const mp_stack* st, const mp_stack::Level& L, const TrainPlan& p, float* f, float* grad,
+                hipStream_t s) {
+  const int H = p.H;
+  const int64_t R = p.rows;
+  const unsigned g0 = (unsigned)(p.np / 16 * (p.wp[0] / 64) / 4);  // exact: np % 128 == 0, wp % 256 == 0
+  if (p.dtype == 1)
+    hipLaunchKernelGGL(k_tw_first<__bf16>, dim3(g0), dim3(256), 0, s, at4(st, p.x4), at4(st, p.w0), p.np, p.wp[0],
+                       reinterpret_cast<__bf16*>(at(st, p.X[0])), reinterpret_cast<__bf16*>(at(st, p.Xb[0])));
+  else if (p.dtype == 2)
+    hipLaunchKernelGGL(k_tw_first<_Float16>, dim3(g0), dim3(256), 0, s, at4(st, p.x4), at4(st, p.w0), p.np, p.wp[0],
+                       reinterpret_cast<_Float16*>(at(st, p.X[0])), reinterpret_cast<_Float16*>(at(st, p.Xb[0])));
   else
-    MP_TRAIN_REDP_DISPATCH(k_train_out_act_bwd4, float, 1, n_tangents, blocks, W, st, Z, s, g, w_last, n, W,
-                           reinterpret_cast<float*>(Abar), static_cast<float*>(nullptr), part);
-  MP_LAUNCHED("k_train_out_act_bwd4");
+    hipLaunchKernelGGL(k_tw_first<float>, dim3(g0), dim3(256), 0, s, at4(st, p.x4), at4(st, p.w0), p.np, p.wp[0],
+                       atf(st, p.X[0]), atf(st, p.Xb[0]));
+  MP_LAUNCHED("k_tw_first");
+  int rc = MP_OK;
+  for (int j = 1; j < H - 1; ++j) {
+    TgArgs a = gemm_args(at(st, p.X[j - 1]), at(st, p.wf[j]), R, p.wp[j], p.wp[j - 1]);
+    a.bias = atf(st, p.bh[j]);
+    a.Z = atf(st, p.Z[j]);
+    a.X = at(st, p.X[j]);
+    a.Xb = at(st, p.Xb[j]);
+    rc = tgemm(p.dtype, TG_ACT, a, s);
+    if (rc != MP_OK) return rc;
+  }
+  if (!f && !grad) return MP_OK;
+  TgArgs a = gemm_args(at(st, p.X[H - 2]), at(st, p.wf[H - 1]), R, p.wp[H - 1], p.wp[H - 2]);
+  a.bias = atf(st, p.bh[H - 1]);
+  a.wl = atf(st, p.wl);
+  a.part = atf(st, p.part_out);
+  rc = tgemm(p.dtype, TG_OUT, a, s);
+  if (rc != MP_OK) return rc;
+  const int slots = p.wp[H - 1] / tgemm_wave_cols(p.dtype);
+  hipLaunchKernelGGL(k_tw_out_sum, dim3(blocks_of(p.n, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(at(st, p.part_out)), slots, p.np, L.biases + p.boff[H], p.n, f,
+                     grad);
+  MP_LAUNCHED("k_tw_out_sum");
   return MP_OK;
 }
 
-}  // extern "C"
-
-namespace mp {
-
-template <class T, class PB, int P>
-static void launch_act_bwd_part_p(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
-                                  int64_t n, void* Abar, void* Ab, float* db_part) {
-  const PB* pb = reinterpret_cast<const PB*>(Pbar);
-  T* a = reinterpret_cast<T*>(Abar);
-  T* ab = reinterpret_cast<T*>(Ab);
-  const dim3 g((unsigned)blocks), b(W / 4);
-  switch (nt) {
-    case 0: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 0, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-    case 1: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 1, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-    case 2: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 2, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-    default: hipLaunchKernelGGL((k_train_act_bwd_part4<T, PB, 3, P>), g, b, 0, st, Z, pb, n, W, a, ab, db_part); break;
-  }
-}
-
-// with the blocked copy: two points per step (whole 16-byte chunks)
-template <class T, class PB>
-static void launch_act_bwd_part(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
-                                int64_t n, void* Abar, void* Ab, float* db_part) {
-  if (Ab && sizeof(T) == 2) launch_act_bwd_part_p<T, PB, 2>(nt, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part);
-  else launch_act_bwd_part_p<T, PB, 1>(nt, blocks, W, st, Z, Pbar, n, Abar, nullptr, db_part);
-}
-
+}  // namespace
 }  // namespace mp
 
 extern "C" {
 
-int mp_train_act_bwd_part(const float* Z, const void* Pbar, int p_dtype, int64_t n, int W, int n_tangents,
-                          void* Abar, int a_dtype, void* Ab, float* db_part, void* stream) {
-  MP_REQUIRE(Z && Pbar && Abar && db_part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_act_bwd_part: bad arguments");
-  MP_REQUIRE(a_dtype >= 0 && a_dtype <= 2 && p_dtype >= 0 && p_dtype <= 2, MP_EINVAL,
-             "mp_train_act_bwd_part: a_dtype %d, p_dtype %d", a_dtype, p_dtype);
-  MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED,
-             "mp_train_act_bwd_part: width %d (needs a multiple of 4, <= 512)", W);
-  MP_REQUIRE(!Ab || (n_tangents == 3 && n % 2 == 0 && a_dtype != 0), MP_EINVAL,
-             "mp_train_act_bwd_part: the blocked copy needs 16-bit A, 3 tangents and even n");
-  if (n == 0) return MP_OK;
-  const int64_t blocks = mp_train_out_bwd_blocks(n);
-  hipStream_t st = as_stream(stream);
-#define MP_ACT_BWD_P(T)                                                                                     \
-  do {                                                                                                    \
-    if (p_dtype == 1) launch_act_bwd_part<T, __bf16>(n_tangents, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part);    \
-    else if (p_dtype == 2) launch_act_bwd_part<T, _Float16>(n_tangents, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part); \
-    else launch_act_bwd_part<T, float>(n_tangents, blocks, W, st, Z, Pbar, n, Abar, Ab, db_part);          \
-  } while (0)
-  if (a_dtype == 1) MP_ACT_BWD_P(__bf16);
-  else if (a_dtype == 2) MP_ACT_BWD_P(_Float16);
-  else MP_ACT_BWD_P(float);
-#undef MP_ACT_BWD_P
-  MP_LAUNCHED("k_train_act_bwd_part4");
+int64_t mp_train_param_count(const mp_stack* st, int level) {
+  mp::TrainPlan p;
+  if (mp::plan_training(st, level, 1, p) != MP_OK) return -1;
+  return p.n_params;
+}
+
+int mp_train_fwd(mp_stack* st, int level, const float* x, int64_t n, float* f, float* grad, int keep,
+                 void* stream) {
+  mp::TrainPlan p;
+  int rc = mp::plan_training(st, level, n, p);
+  if (rc != MP_OK) return rc;
+  MP_REQUIRE(x != nullptr, MP_EINVAL, "mp_train_fwd: x is NULL");
+  rc = mp::ensure_train_ws(st, p.total);
+  if (rc != MP_OK) return rc;
+  hipStream_t s = mp::as_stream(stream);
+  st->kept_level = -1;
+  const mp_stack::Level& L = st->lv[level];
+  rc = mp::stage_inputs(st, L, p, x, s);
+  if (rc != MP_OK) return rc;
+  rc = mp::run_forward(st, L, p, f, grad, s);
+  if (rc != MP_OK) return rc;
+  if (keep) {
+    st->kept_level = level;
+    st->kept_x = x;
+    st->kept_n = n;
+    st->kept_version = st->version;
+  }
   return MP_OK;
 }
 
-int64_t mp_train_first_bwd_blocks(int64_t n) { return (n + RED_ROWS - 1) / RED_ROWS; }
-
-}  // extern "C"
-
-namespace mp {
-
-template <class P>
-static void launch_first_bwd(int nt, int64_t blocks, int W, hipStream_t st, const float* Z, const void* Pbar,
-                             const float* x, const float* v, int64_t n, float* part) {
-  const P* pb = reinterpret_cast<const P*>(Pbar);
-  const dim3 g((unsigned)blocks), b(W / 4);
-  switch (nt) {
-    case 0: hipLaunchKernelGGL((k_train_first_bwd4<P, 0>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
-    case 1: hipLaunchKernelGGL((k_train_first_bwd4<P, 1>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
-    case 2: hipLaunchKernelGGL((k_train_first_bwd4<P, 2>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
-    default: hipLaunchKernelGGL((k_train_first_bwd4<P, 3>), g, b, 0, st, Z, pb, x, v, n, W, part); break;
+int mp_train_bwd(mp_stack* st, int level, const float* x, int64_t n, const float* dL_df, const float* dL_dgrad,
+                 float* dparams, void* stream) {
+  using namespace mp;
+  TrainPlan p;
+  int rc = plan_training(st, level, n, p);
+  if (rc != MP_OK) return rc;
+  MP_REQUIRE(x != nullptr && dparams != nullptr, MP_EINVAL, "mp_train_bwd: x and dparams must be non-NULL");
+  MP_REQUIRE(dL_df != nullptr || dL_dgrad != nullptr, MP_EINVAL, "mp_train_bwd: no seeds (dL_df and dL_dgrad NULL)");
+  hipStream_t s = as_stream(stream);
+  const mp_stack::Level& L = st->lv[level];
+  const bool kept = st->train_ws && st->kept_level == level && st->kept_x == x && st->kept_n == n &&
+                    st->kept_version == st->version && st->train_ws_bytes >= p.total;
+  if (!kept) {
+    rc = ensure_train_ws(st, p.total);
+    if (rc != MP_OK) return rc;
+    rc = stage_inputs(st, L, p, x, s);
+    if (rc != MP_OK) return rc;
+    rc = run_forward(st, L, p, nullptr, nullptr, s);
+    if (rc != MP_OK) return rc;
   }
-}
-
-}  // namespace mp
-
-extern "C" {
-
-int mp_train_first_bwd(const float* Z, const void* Pbar, int p_dtype, const float* x, const float* v, int64_t n,
-                       int W, int n_tangents, float* part, void* stream) {
-  MP_REQUIRE(Z && Pbar && x && part && n >= 0 && W > 0 && n_tangents >= 0 && n_tangents <= 3, MP_EINVAL,
-             "mp_train_first_bwd: bad arguments");
-  MP_REQUIRE(p_dtype >= 0 && p_dtype <= 2, MP_EINVAL, "mp_train_first_bwd: p_dtype %d", p_dtype);
-  MP_REQUIRE(n_tangents == 0 || v, MP_EINVAL, "mp_train_first_bwd: tangent directions are NULL");
-  MP_REQUIRE(W % 4 == 0 && W <= 512, MP_EUNSUPPORTED, "mp_train_first_bwd: width %d (needs a multiple of 4, <= 512)", W);
-  if (n == 0) return MP_OK;
-  const int64_t blocks = mp_train_first_bwd_blocks(n);
-  hipStream_t st = as_stream(stream);
-  if (p_dtype == 1) launch_first_bwd<__bf16>(n_tangents, blocks, W, st, Z, Pbar, x, v, n, part);
-  else if (p_dtype == 2) launch_first_bwd<_Float16>(n_tangents, blocks, W, st, Z, Pbar, x, v, n, part);
-  else launch_first_bwd<float>(n_tangents, blocks, W, st, Z, Pbar, x, v, n, part);
-  MP_LAUNCHED("k_train_first_bwd4");
+  const int H = p.H;
+  const int64_t R = p.rows;
+  const int64_t WR = R / tgemm_wave_rows(p.dtype);  // partial-sum rows of the column epilogues
+  float* tmp = atf(st, p.tmp);
+  float* part = atf(st, p.part_col);
+  float* dW = dparams;                  // weights of every layer, then
+  float* dB = dparams + p.woff[H + 1];  // the biases
+  hipLaunchKernelGGL(k_tw_seeds, dim3(blocks_of(p.np, 256)), dim3(256), 0, s, dL_df, dL_dgrad, n, at4(st, p.sg), p.np);
+  MP_LAUNCHED("k_tw_seeds");
+  // b_out: the sum of the value seeds
+  hipLaunchKernelGGL(k_tw_sum, dim3(1), dim3(1024), 0, s, atf(st, p.sg), p.np, 4, dB + p.boff[H]);
+  MP_LAUNCHED("k_tw_sum");
+  // layer H-1 recomputed, through the output layer: Abar_{H-1}, db_{H-1}, dw_out
+  {
+    TgArgs a = gemm_args(at(st, p.X[H - 2]), at(st, p.wf[H - 1]), R, p.wp[H - 1], p.wp[H - 2]);
+    a.bias = atf(st, p.bh[H - 1]);
+    a.wl = atf(st, p.wl);
+    a.pt4 = at4(st, p.sg);
+    a.X = at(st, p.A[0]);
+    a.Xb = at(st, p.Ab[0]);
+    a.part = part;
+    rc = tgemm(p.dtype, TG_OUT_BWD, a, s);
+    if (rc != MP_OK) return rc;
+    const int64_t C = 2 * (int64_t)p.wp[H - 1];
+    rc = reduce_parts(part, WR, C, tmp, 1, p.w[H - 1], 0, 1, 0, dB + p.boff[H - 1], 0, s);
+    if (rc != MP_OK) return rc;
+    rc = reduce_parts(part, WR, C, tmp, 1, p.w[H - 1], 0, 1, p.wp[H - 1], dW + p.woff[H], 0, s);
+    if (rc != MP_OK) return rc;
+  }
+  int cur = 0;
+  for (int j = H - 1; j >= 1; --j) {
+    // dW_j = Abar_j^T [p; pd]_{j-1} over all rows (split-K partials)
+    {
+      TgArgs a = gemm_args(at(st, p.Ab[cur]), at(st, p.Xb[j - 1]), p.wp[j], p.wp[j - 1], R);
+      a.part = atf(st, p.part_dw);
+      a.splits = p.dw_chunks[j];
+      rc = tgemm(p.dtype, TG_PART, a, s);
+      if (rc != MP_OK) return rc;
+      rc = reduce_parts(atf(st, p.part_dw), p.dw_chunks[j], (int64_t)p.wp[j] * p.wp[j - 1], tmp, p.w[j], p.w[j - 1],
+                        p.wp[j - 1], 1, 0, dW + p.woff[j], p.w[j - 1], s);
+      if (rc != MP_OK) return rc;
+    }
+    // [pbar; pdbar]_{j-1} = [abar; adbar]_j W_j, then through layer j-1's activation
+    TgArgs a = gemm_args(at(st, p.A[cur]), at(st, p.wt[j]), R, p.wp[j - 1], p.wp[j]);
+    a.part = part;
+    if (j >= 2) {
+      a.Z = atf(st, p.Z[j - 1]);
+      a.X = at(st, p.A[1 - cur]);
+      a.Xb = at(st, p.Ab[1 - cur]);
+      rc = tgemm(p.dtype, TG_ACT_BWD, a, s);
+      if (rc != MP_OK) return rc;
+      rc = reduce_parts(part, WR, p.wp[j - 1], tmp, 1, p.w[j - 1], 0, 1, 0, dB + p.boff[j - 1], 0, s);
+      if (rc != MP_OK) return rc;
+      cur = 1 - cur;
+    } else {
+      // layer 0 with its parameter gradients: part [WR][4][wp0]
+      a.pt4 = at4(st, p.x4);
+      a.w0 = at4(st, p.w0);
+      rc = tgemm(p.dtype, TG_FIRST_BWD, a, s);
+      if (rc != MP_OK) return rc;
+      const int64_t C = 4 * (int64_t)p.wp[0];
+      rc = reduce_parts(part, WR, C, tmp, p.w[0], 3, 1, p.wp[0], 0, dW, 3, s);
+      if (rc != MP_OK) return rc;
+      rc = reduce_parts(part, WR, C, tmp, 1, p.w[0], 0, 1, 3 * (int64_t)p.wp[0], dB, 0, s);
+      if (rc != MP_OK) return rc;
+    }
+  }
   return MP_OK;
 }
 

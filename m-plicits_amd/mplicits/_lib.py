@@ -138,37 +138,11 @@ SIGNATURES = {
     "mp_trace_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "mp_trace_batch_stats": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, ctypes.c_void_p]),
     "mp_trace_batch_destroy": (None, [ctypes.c_void_p]),
-    "mp_train_first": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_act": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
-                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                                    ctypes.c_void_p]),
-    "mp_train_act_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                                        ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_dw_chunks": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
-    "mp_train_dw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_seed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
-                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_out_fwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                        ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_out_act_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_act_bwd_part": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
-                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
-                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_out_bwd_blocks": (ctypes.c_int64, [ctypes.c_int64]),
-    "mp_train_first_bwd_blocks": (ctypes.c_int64, [ctypes.c_int64]),
-    "mp_train_first_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
-                                          ctypes.c_void_p, ctypes.c_void_p]),
-    "mp_train_out_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "mp_train_param_count": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int]),
+    "mp_train_fwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "mp_train_bwd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mp_mesh_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "mp_mesh_destroy": (None, [ctypes.c_void_p]),

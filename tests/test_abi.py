@@ -64,9 +64,10 @@ def test_training_data_and_renderer_arguments_are_checked_on_the_host(lib):
     assert rc == _lib.MP_EINVAL and b"seventh column" in lib.mp_last_error()
     rc = lib.mp_neighborhood_samples(dummy, 10, 6, None, dummy, 11, 0.02, 5e-4, 0, 1, dummy, dummy, dummy, None)
     assert rc == _lib.MP_EINVAL and b"without ids" in lib.mp_last_error()
-    rc = lib.mp_train_dw(dummy, dummy, 96, 256, 256, 1, dummy, None)
-    assert rc == _lib.MP_EINVAL and b"multiple of 64" in lib.mp_last_error()
-    rc = lib.mp_train_dw(dummy, dummy, 128, 256, 256, 0, dummy, None)
-    assert rc == _lib.MP_EINVAL and b"bfloat16 (1) or float16 (2)" in lib.mp_last_error()
+    assert lib.mp_train_param_count(None, 0) == -1
+    rc = lib.mp_train_fwd(None, 0, dummy, 16, dummy, dummy, 1, None)
+    assert rc == _lib.MP_EINVAL and b"stack is NULL" in lib.mp_last_error()
+    rc = lib.mp_train_bwd(None, 0, dummy, 16, dummy, None, dummy, None)
+    assert rc == _lib.MP_EINVAL and b"stack is NULL" in lib.mp_last_error()
     rc = lib.mp_render(None, dummy, 1, None, dummy, None, None, None, None, None)
     assert rc == _lib.MP_EINVAL and b"NULL argument" in lib.mp_last_error()

@@ -49,10 +49,10 @@ def _oracle_grads(coarse, med, fine, x, gt, nrm):
     return loss.item(), [g * oms[i // 2] for i, g in enumerate(grads)]
 
 
-@pytest.mark.parametrize("precision,tol,fine_hidden", [("fp32", 2e-3, (64, 64, 64)), ("bf16", 6e-2, (64, 64, 64)),
-                                                       ("fp32", 2e-3, (50, 30, 50))])
+@pytest.mark.parametrize("precision,tol,fine_hidden", [("fp32", 1e-4, (64, 64, 64)), ("bf16", 6e-2, (64, 64, 64)),
+                                                       ("fp32", 1e-4, (50, 30, 50))])
 def test_dropin_residual_loss_gradients(precision, tol, fine_hidden):
-    # (50, 30, 50): widths that are no multiple of 4 take the scalar kernels
+    # (50, 30, 50): ragged widths (padded to whole GEMM tiles)
     coarse, med, fine = _models(fine_hidden)
     for m in (coarse, med, fine):
         m.precision = precision
@@ -96,57 +96,118 @@ def test_fused_step_equals_dropin_path():
         assert torch.allclose(a, p.grad, rtol=1e-3, atol=1e-6 * a.abs().max().item())
 
 
-def test_forward_tangents_are_the_input_gradient():
+def test_level_forward_is_the_input_gradient():
     _, _, fine = _models((64, 32, 64))
     x = (torch.rand(777, 3) * 2 - 1).cuda()
-    layers = training.baked_layers(fine)
-    f, fd, _ = training.forward_tangents(layers, x, training.unit_tangents(777, x.device))
+    f, fd = training.level_forward(fine, x)
     fr, gr = osiren.forward_grad(osiren.layers_from_model(fine), x.double().cpu().numpy())
     assert np.abs(f.double().cpu().numpy() - fr).max() < 1e-5 * max(1, np.abs(fr).max())
     assert np.abs(fd.double().cpu().numpy() - gr).max() < 1e-4 * np.abs(gr).max()
 
 
-def _blocked(t):
-    """[rows, W] -> K-blocked copy: element (r, u) at ((r / 8) * W + u) * 8 + r % 8."""
-    rows, w = t.shape
-    return t.view(rows // 8, 8, w).transpose(1, 2).contiguous().view(-1)
-
-
-@pytest.mark.parametrize("dtype,code", [(torch.bfloat16, 1), (torch.float16, 2)])
-@pytest.mark.parametrize("rows,wout,win", [(64, 256, 256), (12224, 512, 256),
-                                           (1 << 16, 256, 512)])
-def test_split_k_weight_gradient_matches_fp32_gemm(dtype, code, rows, wout, win):
+def _abi_train(st, x, s, g, keep):
+    """mp_train_fwd / mp_train_bwd called through the C ABI."""
     from mplicits import _lib
     lib = _lib.load()
-    g = torch.Generator(device="cuda").manual_seed(rows + wout)
-    A = torch.randn(rows, wout, device="cuda", generator=g).to(dtype)
-    X = torch.randn(rows, win, device="cuda", generator=g).to(dtype)
-    chunks = int(lib.mp_train_dw_chunks(rows, wout, win))
-    part = torch.full((chunks, wout, win), float("nan"), device="cuda")
-    Ab, Xb = _blocked(A), _blocked(X)  # held until the kernel has run
-    _lib.check(lib.mp_train_dw(_lib.ptr(Ab), _lib.ptr(Xb), rows, wout, win, code,
-                               _lib.ptr(part), _lib.stream_handle()))
+    n = x.shape[0]
+    f = torch.empty(n, device="cuda")
+    fd = torch.empty(n, 3, device="cuda")
+    print(f"mp_train_fwd/bwd: n={n} keep={keep}", flush=True)
+    _lib.check(lib.mp_train_fwd(st._handle, 0, _lib.ptr(x), n, _lib.ptr(f), _lib.ptr(fd), keep, _lib.stream_handle()))
+    dp = torch.full((int(lib.mp_train_param_count(st._handle, 0)),), float("nan"), device="cuda")
+    _lib.check(lib.mp_train_bwd(st._handle, 0, _lib.ptr(x), n, _lib.ptr(s), _lib.ptr(g), _lib.ptr(dp),
+                                _lib.stream_handle()))
     torch.cuda.synchronize()
-    got = part.sum(0)
-    ref = A.float().t() @ X.float()
-    # same products, FP32 accumulation in a different order
-    bound = 1e-4 * (A.float().abs().t() @ X.float().abs()) + 1e-6
-    assert torch.all((got - ref).abs() <= bound), float(((got - ref).abs() / bound).max())
+    return f, fd, dp
 
 
-def test_split_k_weight_gradient_rejects_ragged_rows():
+@pytest.mark.parametrize("hidden,precision,tol", [((512, 512, 512), "fp32", 1e-4), ((100, 200, 50), "fp32", 1e-4),
+                                                  ((256, 256), "fp32", 1e-4), ((64, 64, 64, 64), "fp32", 1e-4),
+                                                  ((512, 512, 512), "bf16", 3e-2), ((256, 256, 256), "fp16", 1e-2)])
+def test_train_entry_points_against_fp64_autograd(hidden, precision, tol):
+    """f, grad f and the parameter gradients of sum [s f + g . grad f] (the
+    seeds of any loss on f and grad f) from the C ABI against FP64 autograd
+    through the oracle's SIREN, with the create_graph gradient: max error per
+    parameter tensor relative to its max."""
+    m = seeded_siren(list(hidden), 90.0, seed=70 + len(hidden), last_scale=0.1).cuda()
+    st = LevelStack([m], precision)
+    st.refresh()
+    gen = torch.Generator().manual_seed(5)
+    n = 3001  # not a multiple of the point padding
+    x = (torch.rand(n, 3, generator=gen) * 2 - 1).cuda()
+    s = torch.randn(n, generator=gen).cuda()
+    g = torch.randn(n, 3, generator=gen).cuda()
+    f, fd, dp = _abi_train(st, x, s, g, keep=1)
+    lay = [(torch.from_numpy(w).requires_grad_(True), torch.from_numpy(b).requires_grad_(True))
+           for w, b in osiren.layers_from_model(m)]
+    fr, gr = olosses.level_value_grad(lay, x.double().cpu(), True)
+    obj = (s.double().cpu()[:, None] * fr).sum() + (g.double().cpu() * gr).sum()
+    ref = torch.autograd.grad(obj, [w for w, _ in lay] + [b for _, b in lay])
+    ftol = 1e-5 if precision == "fp32" else 3e-3
+    assert (f.double().cpu() - fr[:, 0].detach()).abs().max() <= ftol * max(1.0, fr.abs().max().item())
+    assert (fd.double().cpu() - gr.detach()).abs().max() <= 10 * ftol * gr.abs().max().item()
+    assert not torch.isnan(dp).any()
+    off = 0
+    for r in ref:
+        got = dp[off:off + r.numel()].double().cpu().view_as(r)
+        off += r.numel()
+        err = ((got - r).abs().max() / r.abs().max()).item()
+        assert err <= tol, (tuple(r.shape), err)
+    assert off == dp.numel()
+    # recomputed activations (no keep) give the same gradients, bit for bit
+    _, _, dp2 = _abi_train(st, x, s, g, keep=0)
+    assert torch.equal(dp, dp2)
+
+
+def test_train_entry_points_check_their_arguments():
     from mplicits import _lib
     lib = _lib.load()
-    A = torch.zeros(96, 256, device="cuda", dtype=torch.bfloat16)
-    part = torch.empty(int(lib.mp_train_dw_chunks(96, 256, 256)), 256, 256, device="cuda")
-    with pytest.raises(ValueError, match="multiple of 64"):
-        _lib.check(lib.mp_train_dw(_lib.ptr(A), _lib.ptr(A), 96, 256, 256, 1, _lib.ptr(part), _lib.stream_handle()))
+    m = seeded_siren([64], 30.0, seed=3).cuda()  # one sine layer: no hidden GEMM to train through
+    st = LevelStack([m], "fp32")
+    x = torch.zeros(8, 3, device="cuda")
+    rc = lib.mp_train_fwd(st._handle, 0, _lib.ptr(x), 8, None, None, 0, _lib.stream_handle())
+    assert rc == _lib.MP_EUNSUPPORTED and b"sine layers" in lib.mp_last_error()
+    m2 = seeded_siren([64, 64], 30.0, seed=4).cuda()
+    st2 = LevelStack([m2], "bf16x3")
+    rc = lib.mp_train_fwd(st2._handle, 0, _lib.ptr(x), 8, None, None, 0, _lib.stream_handle())
+    assert rc == _lib.MP_EUNSUPPORTED and b"three-term" in lib.mp_last_error()
+    rc = lib.mp_train_fwd(st2._handle, 1, _lib.ptr(x), 8, None, None, 0, _lib.stream_handle())
+    assert rc == _lib.MP_EINVAL
 
 
-def test_wide_level_gradients_take_the_blocked_path():
+@pytest.mark.parametrize("precision,tol", [("bf16", 3e-2), ("fp32", 1e-4)])
+def test_train_entry_points_at_many_tiles(precision, tol):
+    """40000 points (160000 activation rows: many row tiles per GEMM and many
+    split-K chunks per weight gradient), 512-wide layers, against FP64
+    autograd of the oracle's SIREN run on the GPU."""
+    m = seeded_siren([512, 512, 512], 90.0, seed=91, last_scale=0.1).cuda()
+    st = LevelStack([m], precision)
+    st.refresh()
+    gen = torch.Generator().manual_seed(9)
+    n = 40000
+    x = (torch.rand(n, 3, generator=gen) * 2 - 1).cuda()
+    s = torch.randn(n, generator=gen).cuda()
+    g = torch.randn(n, 3, generator=gen).cuda()
+    f, fd, dp = _abi_train(st, x, s, g, keep=1)
+    lay = [(torch.from_numpy(w).cuda().requires_grad_(True), torch.from_numpy(b).cuda().requires_grad_(True))
+           for w, b in osiren.layers_from_model(m)]
+    fr, gr = olosses.level_value_grad(lay, x.double(), True)
+    obj = (s.double()[:, None] * fr).sum() + (g.double() * gr).sum()
+    ref = torch.autograd.grad(obj, [w for w, _ in lay] + [b for _, b in lay])
+    ftol = 1e-5 if precision == "fp32" else 3e-3
+    assert (f.double() - fr[:, 0].detach()).abs().max() <= ftol * max(1.0, fr.abs().max().item())
+    assert (fd.double() - gr.detach()).abs().max() <= 10 * ftol * gr.abs().max().item()
+    off = 0
+    for r in ref:
+        got = dp[off:off + r.numel()].double().view_as(r)
+        off += r.numel()
+        err = ((got - r).abs().max() / r.abs().max()).item()
+        assert err <= tol, (tuple(r.shape), err)
+
+
+def test_wide_level_gradients_bf16():
     coarse, med, fine = _models((256, 256, 256))
     x, gt, nrm = _batch(2048, 3)
-    assert training._blocked_dw("bf16", 2048, 3, 256, 256)
     ref_loss, ref = _oracle_grads(coarse, med, fine, x, gt, nrm)
     fine.precision = "bf16"
     for m in (coarse, med):
