@@ -104,7 +104,10 @@ int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, 
  * create_mesh_multistage does with deltas=[] (i3d/meshing.py:232-233).
  * If grad != NULL, also writes the input gradient [n][3] of that sum
  * (i3d/diff_operators.py:259-265; renderer Normals::inference,
- * renderer/cuda/gemms.cu:263-337). */
+ * renderer/cuda/gemms.cu:263-337); both then come from one gradient pass per
+ * level, whose forward rounds like the value-only kernels but may
+ * accumulate in another order (values equal to within the precision's
+ * rounding, not bit for bit). */
 int mp_eval(const mp_stack* stack, const float* xyz, int64_t n, int lvl0, int lvl1, float* sdf,
             float* grad, void* stream);
 

@@ -39,8 +39,10 @@ int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n
 int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t n, float* f, float* g,
                          int accumulate, hipStream_t stream, const unsigned long long* n_dev = nullptr);
 
-// Input gradient of sum_{l=lvl0}^{lvl1-1} f_l at n points (overwrites grad).
-int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
+// Input gradient of sum_{l=lvl0}^{lvl1-1} f_l at n points (overwrites grad),
+// and the sum itself into sdf unless it is NULL (the gradient kernels run each
+// level's forward anyway).
+int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* sdf, float* grad,
                 hipStream_t stream);
 
 // The band-culled extraction pipeline behind mp_extract_volume.  scratch ==

@@ -282,10 +282,10 @@ int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n
   return MP_EUNSUPPORTED;
 }
 
-int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* grad,
+int launch_grad(const mp_stack* st, int lvl0, int lvl1, const float* xyz, int64_t n, float* sdf, float* grad,
                 hipStream_t stream) {
   for (int l = lvl0; l < lvl1; ++l) {
-    int rc = launch_level_grad(st, l, xyz, n, nullptr, grad, l > lvl0, stream);
+    int rc = launch_level_grad(st, l, xyz, n, sdf, grad, l > lvl0, stream);
     if (rc != MP_OK) return rc;
   }
   return MP_OK;

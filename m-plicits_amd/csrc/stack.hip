@@ -335,6 +335,8 @@ int mp_eval(const mp_stack* st, const float* xyz, int64_t n, int lvl0, int lvl1,
              "mp_eval: need 0 <= lvl0 < lvl1 <= %d, got [%d, %d)", st->n_levels, lvl0, lvl1);
   if (n == 0) return MP_OK;
   MP_REQUIRE(xyz != nullptr && sdf != nullptr, MP_EINVAL, "mp_eval: xyz and sdf must be non-NULL");
+  // with gradients one pass per level gives both
+  if (grad) return launch_grad(st, lvl0, lvl1, xyz, n, sdf, grad, as_stream(stream));
   for (int l = lvl0; l < lvl1; ++l) {
     EvalTarget t{};
     t.src = SRC_POINTS;
@@ -343,10 +345,6 @@ int mp_eval(const mp_stack* st, const float* xyz, int64_t n, int lvl0, int lvl1,
     t.out = sdf;
     t.accumulate = l > lvl0;
     int rc = launch_level(st, l, t, as_stream(stream));
-    if (rc != MP_OK) return rc;
-  }
-  if (grad) {
-    int rc = launch_grad(st, lvl0, lvl1, xyz, n, grad, as_stream(stream));
     if (rc != MP_OK) return rc;
   }
   return MP_OK;

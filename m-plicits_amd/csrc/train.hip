@@ -26,10 +26,12 @@
 //             TG_FIRST_BWD (j = 1, layer 0's backward and parameter gradients)
 // Stored between them: the unit-blocked activations X_j (GEMM A operands) and
 // their row-blocked copies Xb_j (weight-gradient operands) of layers
-// 0..H-2, and the FP32 pre-activations Z_j of layers 1..H-2.  Layer 0's and
-// layer H-1's pre-activations are recomputed where they are needed (three
-// FMAs per unit for layer 0; the layer H-1 GEMM in the backward), which
-// halves the HBM traffic of those layers.  Widths are padded to 256 and
+// 0..H-2, and for layers 1..H-2 the derivative factors D_j = (cos a,
+// sin(a) ad_t) that the backward through the activation needs, in the
+// operand type (half the traffic of FP32 pre-activations in 16 bit,
+// and no sine or cosine in the backward).  Layer 0's and layer H-1's
+// pre-activations are recomputed where they are needed (three FMAs per unit
+// for layer 0; the layer H-1 GEMM in the backward).  Widths are padded to 256 and
 // points to 128 (zero weights, zero seeds), so the GEMMs need no edge
 // handling; the padded units and points contribute nothing.
 #include "common.h"
@@ -54,8 +56,8 @@ struct TrainPlan {
   // workspace offsets (bytes)
   size_t x4 = 0, w0 = 0, wl = 0, sg = 0;
   size_t bh[MAX_LAYERS] = {}, wf[MAX_LAYERS] = {}, wt[MAX_LAYERS] = {};
-  size_t Z[MAX_LAYERS] = {}, X[MAX_LAYERS] = {}, Xb[MAX_LAYERS] = {};
-  size_t A[2] = {}, Ab[2] = {}, part_out = 0, part_col = 0, part_dw = 0, tmp = 0;
+  size_t D[MAX_LAYERS] = {}, X[MAX_LAYERS] = {}, Xb[MAX_LAYERS] = {};
+  size_t A[2] = {}, Ab[2] = {}, part_out = 0, part_col = 0, part_dw = 0, tmp = 0, red = 0;
   size_t total = 0;
   int dw_chunks[MAX_LAYERS] = {};
 };
@@ -111,7 +113,7 @@ int plan_training(const mp_stack* st, int level, int64_t n, TrainPlan& p) {
   for (int k = 0; k < H - 1; ++k) {
     p.X[k] = take((size_t)R * p.wp[k] * p.esize);
     p.Xb[k] = take((size_t)R * p.wp[k] * p.esize);
-    if (k >= 1) p.Z[k] = take((size_t)R * p.wp[k] * 4);
+    if (k >= 1) p.D[k] = take((size_t)R * p.wp[k] * p.esize);
   }
   for (int k = 1; k < H; ++k) wmax = p.wp[k] > wmax ? p.wp[k] : wmax;
   for (int c = 0; c < 2; ++c) {
@@ -134,6 +136,7 @@ int plan_training(const mp_stack* st, int level, int64_t n, TrainPlan& p) {
   }
   p.part_dw = take(dw);
   p.tmp = take((size_t)TMP_FLOATS * 4);
+  p.red = take(256 * 4);
   p.total = off;
   return MP_OK;
 }
@@ -283,20 +286,22 @@ __global__ void k_tw_out_sum(const float4* __restrict__ part, int slots, int64_t
   }
 }
 
-// out[0] = sum_i src[i * stride] for i < n (one workgroup)
-__global__ __launch_bounds__(1024) void k_tw_sum(const float* __restrict__ src, int64_t n, int stride,
-                                                 float* __restrict__ out) {
-  __shared__ float red[1024];
+// out[b] = sum of src[i * stride] over the stripe i = b, b + gridDim.x, ...
+// of i < n (workgroup b; 256 threads)
+__global__ __launch_bounds__(256) void k_tw_sum(const float* __restrict__ src, int64_t n, int stride,
+                                                float* __restrict__ out) {
+  __shared__ float red[256];
   float acc = 0.f;
-#pragma unroll 8
-  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += src[i * stride];
+  const int64_t step = (int64_t)gridDim.x * 256;
+#pragma unroll 4
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += step) acc += src[i * stride];
   red[threadIdx.x] = acc;
   __syncthreads();
-  for (int h = 512; h > 0; h >>= 1) {
+  for (int h = 128; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // tmp[slice][c] = sum over parts p = slice, slice + S, ... of part[p][c]
@@ -318,6 +323,7 @@ __global__ void k_gather_sum(const float* __restrict__ tmp, int S, int64_t C, in
   const int r = (int)(e / cols), c = (int)(e % cols);
   const int64_t src = off + r * rs + c * cs;
   float acc = 0.f;
+#pragma unroll 8
   for (int sl = 0; sl < S; ++sl) acc += tmp[(int64_t)sl * C + src];
   out[(int64_t)r * out_rs + c] = acc;
 }
@@ -400,7 +406,7 @@ int run_forward(const mp_stack* st, const mp_stack::Level& L, const TrainPlan& p
   for (int j = 1; j < H - 1; ++j) {
     TgArgs a = gemm_args(at(st, p.X[j - 1]), at(st, p.wf[j]), R, p.wp[j], p.wp[j - 1]);
     a.bias = atf(st, p.bh[j]);
-    a.Z = atf(st, p.Z[j]);
+    a.D = at(st, p.D[j]);
     a.X = at(st, p.X[j]);
     a.Xb = at(st, p.Xb[j]);
     rc = tgemm(p.dtype, TG_ACT, a, s);
@@ -486,7 +492,8 @@ int mp_train_bwd(mp_stack* st, int level, const float* x, int64_t n, const float
   hipLaunchKernelGGL(k_tw_seeds, dim3(blocks_of(p.np, 256)), dim3(256), 0, s, dL_df, dL_dgrad, n, at4(st, p.sg), p.np);
   MP_LAUNCHED("k_tw_seeds");
   // b_out: the sum of the value seeds
-  hipLaunchKernelGGL(k_tw_sum, dim3(1), dim3(1024), 0, s, atf(st, p.sg), p.np, 4, dB + p.boff[H]);
+  hipLaunchKernelGGL(k_tw_sum, dim3(256), dim3(256), 0, s, atf(st, p.sg), p.np, 4, atf(st, p.red));
+  hipLaunchKernelGGL(k_tw_sum, dim3(1), dim3(256), 0, s, atf(st, p.red), (int64_t)256, 1, dB + p.boff[H]);
   MP_LAUNCHED("k_tw_sum");
   // layer H-1 recomputed, through the output layer: Abar_{H-1}, db_{H-1}, dw_out
   {
@@ -522,7 +529,7 @@ int mp_train_bwd(mp_stack* st, int level, const float* x, int64_t n, const float
     TgArgs a = gemm_args(at(st, p.A[cur]), at(st, p.wt[j]), R, p.wp[j - 1], p.wp[j]);
     a.part = part;
     if (j >= 2) {
-      a.Z = atf(st, p.Z[j - 1]);
+      a.D = at(st, p.D[j - 1]);
       a.X = at(st, p.A[1 - cur]);
       a.Xb = at(st, p.Ab[1 - cur]);
       rc = tgemm(p.dtype, TG_ACT_BWD, a, s);

@@ -7,7 +7,7 @@ namespace mp {
 
 // epilogues
 enum {
-  TG_ACT = 0,        // forward hidden layer: Z, X = [sin; cos * tangent] unit-blocked, Xb row-blocked
+  TG_ACT = 0,        // forward hidden layer: D, X = [sin; cos * tangent] unit-blocked, Xb row-blocked
   TG_OUT = 1,        // forward last sine layer + output layer: (f, grad f) partial sums per column range
   TG_OUT_BWD = 2,    // last sine layer recomputed + output-layer backward: Abar, Ab, (db, dw_out) partials
   TG_ACT_BWD = 3,    // backward through a hidden layer's activation: Abar, Ab, db partials
@@ -31,7 +31,8 @@ struct TgArgs {
   const float* wl;    // TG_OUT, TG_OUT_BWD: output-layer weights [N]
   const float4* pt4;  // TG_OUT_BWD: seeds (dL/df, dL/dgrad f) per point; TG_FIRST_BWD: (x, y, z, 0) per point
   const float4* w0;   // TG_FIRST_BWD: (W_0[u][0..2], b_0[u]) per unit
-  float* Z;           // TG_ACT: out, TG_ACT_BWD: in; pre-activations, point-major [M/4][N][4] FP32
+  void* D;            // TG_ACT: out, TG_ACT_BWD: in; the activation's derivative factors
+                      // (cos a, sin(a) ad_1..3), point-major [M/4][N][4], operand type
   void* X;            // TG_ACT, TG_OUT_BWD, TG_ACT_BWD: out, unit-blocked [N/KL][M][KL]
   void* Xb;           // same as X, row-blocked [M/KL][N][KL] (NULL: not written)
   float* part;        // partial sums, see tgemm_part_floats

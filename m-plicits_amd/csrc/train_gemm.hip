@@ -22,13 +22,17 @@
 // with two DPP quad permutations, after which lane t of the quad holds row t
 // of its point for the four units (an 8-byte or 16-byte store, 128 contiguous
 // bytes per 16-bit chunk over the two lane groups).  The row-blocked copy and
-// the FP32 pre-activations Z (point-major [points][N][4]) are stored straight
-// from the C layout.
+// the activation's derivative factors D (point-major [points][N][4] in the
+// operand type: cos a and sin(a) ad_t, all the backward needs of the layer)
+// are stored straight from the C layout.
 //
 // Workgroup: four waves in 2 x 2, each a 4 x 4 grid of MFMA blocks (16-bit:
-// 32x32x16, 128 x 128 per wave; FP32: 16x16x4, 64 x 64 per wave).  Operands
-// stream through a TG_RING-deep register ring, one chunk per block row and
-// column and K step of 16.  FP32 blocks take a chunk of four consecutive k per
+// 32x32x16, 128 x 128 per wave; FP32: 16x16x4, 64 x 64 per wave).  Per K step
+// of 16 the workgroup's operand tiles (16 KB) are copied once into an LDS
+// ring by direct 16-byte global-to-LDS loads (several steps in flight), and
+// every wave reads its fragments from there, one step ahead of its MFMAs:
+// half the L2 traffic of per-wave register loads, and no registers held by
+// the loads in flight.  FP32 blocks take a chunk of four consecutive k per
 // lane group and run four MFMAs on it: instruction e reduces k = 16 s + 4 g + e
 // over the lane groups g (the reduction order is free when both operands
 // follow it).  The workgroups that read the same operand rows (the column
@@ -36,6 +40,8 @@
 // on one XCD, so those rows are fetched from HBM into its L2 once.
 // No bounds checks: the host pads M, N, K to whole tiles (widths to 256,
 // points to 128).
+#include <type_traits>
+
 #include "common.h"
 #include "mlp_core.h"
 #include "train_gemm.h"
@@ -56,8 +62,16 @@ template <> struct TgTraits<float> {
   typedef S16x16x4_f32 S;
 };
 
-constexpr int TG_RING = 4;
+constexpr int TG_RING = 4;  // K steps per chunk of a split-K range (multiples keep the chunks whole)
+constexpr int TG_STAGE_BYTES = 16384;
+// LDS ring depth: 8 stages (128 KB, one workgroup per CU, as its registers
+// allow anyway) for 16-bit operands; 4 (64 KB) in FP32, whose smaller
+// register footprint lets two workgroups share a CU
+template <class T>
+constexpr int tg_stages() { return sizeof(T) == 4 ? 4 : 8; }
 constexpr int TG_BR = 4, TG_BC = 4;
+
+typedef __attribute__((address_space(3))) void lds_void;
 
 // TgArgs plus the launch geometry
 struct TgLaunch {
@@ -209,45 +223,80 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
 #pragma unroll
       for (int q = 0; q < S::CREGS; ++q) acc[i][j][q] = 0.f;
 
+  // LDS ring of SG K steps; a stage holds the workgroup's A tile (CPS planes
+  // of TM chunks) and B tile (CPS planes of TN chunks), 16 KB, in the order of
+  // the blocked layout, so both the copy and the fragment reads are
+  // contiguous runs of 16-byte chunks.
+  constexpr int SG = tg_stages<T>();
+  __shared__ __attribute__((aligned(16))) char ring[SG * TG_STAGE_BYTES];
   if (s0 < s1) {
-    // chunk (row r, K step s, lane group g) at (s CPS + g) rows + r
+    constexpr int TM = 2 * WM, TN = 2 * WN;
+    constexpr int PP = TM / 64;  // 1 KB pieces per plane (TM == TN)
+    static_assert(TM == TN && CPS * TM * 16 == TG_STAGE_BYTES / 2, "stage = A tile + B tile");
     const int64_t lda = a.M, ldb = a.N;
-    const Chunk* pa = reinterpret_cast<const Chunk*>(a.A) + (s0 * CPS + g) * lda + m0 + col;
-    const Chunk* pb = reinterpret_cast<const Chunk*>(a.B) + (s0 * CPS + g) * ldb + n0 + col;
-    const Chunk* const pa_last = reinterpret_cast<const Chunk*>(a.A) + ((s1 - 1) * CPS + g) * lda + m0 + col;
-    const Chunk* const pb_last = reinterpret_cast<const Chunk*>(a.B) + ((s1 - 1) * CPS + g) * ldb + n0 + col;
-    const int64_t da = CPS * lda, db = CPS * ldb;
-    Chunk ra[TG_RING][TG_BR], rb[TG_RING][TG_BC];
-    auto load = [&](int k) {
+    const int64_t mt0 = mt * TM, nt0 = nt * TN;
+    const Chunk* A = reinterpret_cast<const Chunk*>(a.A);
+    const Chunk* B = reinterpret_cast<const Chunk*>(a.B);
+    // Stage `slot` <- K step ks: 16 pieces of 1 KB (one 16-byte direct load
+    // per lane each), pieces 0..7 the A tile, 8..15 the B tile; wave w copies
+    // pieces 4w..4w+3.
+    auto copy = [&](int64_t ks, int slot) {
 #pragma unroll
-      for (int i = 0; i < TG_BR; ++i) ra[k][i] = pa[i * R];
+      for (int k = 0; k < 4; ++k) {
+        const int piece = 4 * wave + k;
+        const int q = piece & 7, plane = q / PP, r = (q % PP) * 64 + lane;
+        const Chunk* src = piece < 8 ? A + (ks * CPS + plane) * lda + mt0 + r : B + (ks * CPS + plane) * ldb + nt0 + r;
+        __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(ring + slot * TG_STAGE_BYTES + piece * 1024), 16,
+                                         0, 0);
+      }
+    };
+    const int64_t nsteps = s1 - s0;
+    auto clamp = [&](int64_t t) { return s0 + (t < nsteps ? t : nsteps - 1); };
+    // this wave's fragment offsets within a stage (in chunks)
+    const int fa = g * TM + (wave >> 1) * WM + col;
+    const int fb = TM * CPS + g * TN + (wave & 1) * WN + col;
+    Chunk ra[2][TG_BR], rb[2][TG_BC];
+    auto frags = [&](int64_t t, int k) {  // k: a constant after inlining
+      const Chunk* st = reinterpret_cast<const Chunk*>(ring + (t % SG) * TG_STAGE_BYTES);
 #pragma unroll
-      for (int j = 0; j < TG_BC; ++j) rb[k][j] = pb[j * R];
-      // past the last step the ring re-reads it instead of branching
-      pa = pa + da <= pa_last ? pa + da : pa_last;
-      pb = pb + db <= pb_last ? pb + db : pb_last;
-      // keeps the compiler from sinking the ring's loads to their first use
-      asm volatile("" ::: "memory");
+      for (int i = 0; i < TG_BR; ++i) ra[k][i] = st[fa + i * R];
+#pragma unroll
+      for (int j = 0; j < TG_BC; ++j) rb[k][j] = st[fb + j * R];
     };
 #pragma unroll
-    for (int k = 0; k < TG_RING; ++k) load(k);
-    for (int64_t s = s0; s < s1; s += TG_RING) {
+    for (int d = 0; d < SG - 1; ++d) copy(clamp(d), d);
+    // step 0 in: this wave's copies of steps 1..SG-2 may still fly
+    asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(4 * (SG - 2)) : "memory");
+    frags(0, 0);
+    // one K step on fragment buffer K (compile-time, so the buffers stay in registers)
+    auto step = [&](int64_t t, auto kc) {
+      constexpr int K = decltype(kc)::value;
+      // Step t + 1 in (this wave's copies of the SG - 3 steps after it may
+      // still fly; after the barrier every wave's are), and every wave is
+      // done with stage (t - 1) % SG, which the next copy overwrites.  Its
+      // fragments are read while the MFMAs of step t run.
+      asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(4 * (SG - 3)) : "memory");
+      copy(clamp(t + SG - 1), (int)((t + SG - 1) % SG));
+      frags(t + 1 < nsteps ? t + 1 : t, K ^ 1);
 #pragma unroll
-      for (int k = 0; k < TG_RING; ++k) {
+      for (int i = 0; i < TG_BR; ++i)
 #pragma unroll
-        for (int i = 0; i < TG_BR; ++i)
+        for (int j = 0; j < TG_BC; ++j) {
+          if constexpr (sizeof(T) == 4) {
 #pragma unroll
-          for (int j = 0; j < TG_BC; ++j) {
-            if constexpr (sizeof(T) == 4) {
-#pragma unroll
-              for (int e = 0; e < 4; ++e) acc[i][j] = S::mma(ra[k][i][e], rb[k][j][e], acc[i][j]);
-            } else {
-              acc[i][j] = S::mma(ra[k][i], rb[k][j], acc[i][j]);
-            }
+            for (int e = 0; e < 4; ++e) acc[i][j] = S::mma(ra[K][i][e], rb[K][j][e], acc[i][j]);
+          } else {
+            acc[i][j] = S::mma(ra[K][i], rb[K][j], acc[i][j]);
           }
-        load(k);
-      }
+        }
+    };
+    // nsteps is even (a multiple of the split-K granule)
+    for (int64_t t = 0; t < nsteps; t += 2) {
+      step(t, std::integral_constant<int, 0>());
+      step(t + 1, std::integral_constant<int, 1>());
     }
+    // the tail's clamped copies land before the workgroup retires
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   }
 
   // ---- epilogues ----
@@ -289,16 +338,17 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
       // the block row's loads first, so their latencies overlap: per-point
       // inputs (seeds for TG_OUT_BWD, coordinates for TG_FIRST_BWD) and the
       // pre-activations of TG_ACT_BWD
+      typedef T v4 __attribute__((ext_vector_type(4)));
       float4 pv[PTS];
-      float4 zb[TG_BC][PTS];
+      v4 zb[TG_BC][PTS];
 #pragma unroll
       for (int pt = 0; pt < PTS; ++pt) {
         const int64_t p = (m0 + i * R + S::rowmap(g, 4 * pt)) / 4;
         pv[pt] = (EPI == TG_OUT_BWD || EPI == TG_FIRST_BWD) ? a.pt4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int j = 0; j < TG_BC; ++j)
-          zb[j][pt] = EPI == TG_ACT_BWD ? reinterpret_cast<const float4*>(a.Z)[p * N + n0 + j * R + col]
-                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < TG_BC; ++j) {
+          if constexpr (EPI == TG_ACT_BWD) zb[j][pt] = reinterpret_cast<const v4*>(a.D)[p * N + n0 + j * R + col];
+        }
       }
       float fo[PTS * 4];  // TG_OUT: (f, grad f) of the lane's points over its columns
 #pragma unroll
@@ -314,11 +364,10 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
           const int64_t mv = m0 + i * R + S::rowmap(g, q0);  // the point's value row
           const float v0 = acc[i][j][q0], v1 = acc[i][j][q0 + 1], v2 = acc[i][j][q0 + 2], v3 = acc[i][j][q0 + 3];
           if constexpr (EPI == TG_ACT) {
-            // a = W p + b, p' = sin a, pd'_t = cos(a) ad_t
-            const float z = v0 + bias;
+            // a = W p + b, p' = sin a, pd'_t = cos(a) ad_t; D = (cos a, sin(a) ad_t)
             float sn, cs;
-            sincos_reduced(z, sn, cs);
-            reinterpret_cast<float4*>(a.Z)[(mv / 4) * N + n] = make_float4(z, v1, v2, v3);
+            sincos_reduced(v0 + bias, sn, cs);
+            st4rows(reinterpret_cast<T*>(a.D) + ((mv / 4) * N + n) * 4, cs, sn * v1, sn * v2, sn * v3);
             const float x1 = cs * v1, x2 = cs * v2, x3 = cs * v3;
             st_unit_blocked(X, M, n, mv, t4, sn, x1, x2, x3);
             if (Xb) st_row_blocked(Xb, N, n, mv, sn, x1, x2, x3);
@@ -348,10 +397,9 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
             if (Xb) st_row_blocked(Xb, N, n, mv, abar, ad1, ad2, ad3);
           } else if constexpr (EPI == TG_ACT_BWD) {
             // abar = cos(a) pbar - sin(a) sum_t ad_t pdbar_t, adbar_t = cos(a) pdbar_t
-            const float4 zz = zb[j][pt];
-            float sn, cs;
-            sincos_reduced(zz.x, sn, cs);
-            const float abar = fmaf(cs, v0, -sn * fmaf(zz.w, v3, fmaf(zz.z, v2, zz.y * v1)));
+            const v4 d = zb[j][pt];
+            const float cs = (float)d[0];
+            const float abar = fmaf(cs, v0, -fmaf((float)d[3], v3, fmaf((float)d[2], v2, (float)d[1] * v1)));
             const float ad1 = cs * v1, ad2 = cs * v2, ad3 = cs * v3;
             csum[j][0] += abar;
             st_unit_blocked(X, M, n, mv, t4, abar, ad1, ad2, ad3);
@@ -436,7 +484,7 @@ template <class T>
 static int launch_epi(int epi, const TgArgs& a, hipStream_t s) {
   switch (epi) {
     case TG_ACT:
-      MP_REQUIRE(a.Z && a.X, MP_EINVAL, "training GEMM: TG_ACT needs Z and X");
+      MP_REQUIRE(a.D && a.X, MP_EINVAL, "training GEMM: TG_ACT needs D and X");
       return launch_tg<T, TG_ACT>(a, s);
     case TG_OUT:
       MP_REQUIRE(a.wl && a.part, MP_EINVAL, "training GEMM: TG_OUT needs wl and part");
@@ -445,7 +493,7 @@ static int launch_epi(int epi, const TgArgs& a, hipStream_t s) {
       MP_REQUIRE(a.wl && a.pt4 && a.X && a.part, MP_EINVAL, "training GEMM: TG_OUT_BWD needs wl, seeds, X, part");
       return launch_tg<T, TG_OUT_BWD>(a, s);
     case TG_ACT_BWD:
-      MP_REQUIRE(a.Z && a.X && a.part, MP_EINVAL, "training GEMM: TG_ACT_BWD needs Z, X and part");
+      MP_REQUIRE(a.D && a.X && a.part, MP_EINVAL, "training GEMM: TG_ACT_BWD needs D, X and part");
       return launch_tg<T, TG_ACT_BWD>(a, s);
     case TG_FIRST_BWD:
       MP_REQUIRE(a.pt4 && a.w0 && a.part, MP_EINVAL, "training GEMM: TG_FIRST_BWD needs points, W_0 and part");
@@ -486,7 +534,7 @@ int tgemm_weight_grad_chunks(int dtype, int64_t rows, int Wout, int Win) {
   const int tiles = (Wout / tgemm_tile_rows(dtype)) * (Win / tgemm_tile_cols(dtype));
   if (tiles <= 0) return 0;
   const int64_t steps = rows / 16;
-  int64_t chunks = (512 + tiles - 1) / tiles;  // about two workgroups per CU
+  int64_t chunks = (256 + tiles - 1) / tiles;  // one workgroup per CU (the LDS ring allows one)
   chunks = (chunks + 7) / 8 * 8;
   const int64_t max_chunks = steps / TG_RING;
   if (chunks > max_chunks) chunks = max_chunks;
