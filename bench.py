@@ -237,40 +237,46 @@ def cpu_baseline(levels, vol, budget_chunks=512):
     return t_eval, t_mc, len(tri), ids, threads
 
 
-def trace_errors(levels, stack, cams, width, height, r, n=2048, seed=0):
-    """CPU leg: FP64 oracle at n traced hits of r (hit = eye + depth * d with
-    the oracle's ray directions): |sum of the levels| there (the hit's distance
-    to the FP64 surface, bounded by the stopping threshold 1e-4 plus the
-    evaluation error), the evaluation error itself |F_path(x) - F_64(x)| (the
-    stack in the traced precision at the same points), and the angle between
-    the returned normal and the FP64 gradient of the sum."""
+def trace_errors(levels, stack, cams, width, height, r, chunk=1 << 19):
+    """CPU leg (checker): the FP64 oracle at every traced hit of r (hit = eye
+    + depth * d with the oracle's ray directions, rounded to float32): |sum of
+    the levels| there (the hit's distance to the FP64 surface, bounded by the
+    stopping threshold 1e-4 plus the evaluation error), the evaluation error
+    itself |F_path(x) - F_64(x)| (the stack in the traced precision at the
+    same points), and the angle between the returned normal and the FP64
+    gradient of the sum.  The FP64 values come from the oracle's forward-mode
+    chain rule in torch float64 on the device."""
     from oracle import siren as osiren
     from oracle import trace as otrace
     depth = r["depth"].reshape(-1)
     hits = torch.nonzero(depth >= 0).flatten()
     if len(hits) == 0:
         return None
-    pick = hits[torch.randperm(len(hits), generator=torch.Generator().manual_seed(seed))[:n].to(hits.device)]
-    ids = pick.cpu().numpy()
-    d = depth[pick].double().cpu().numpy()
-    nrm = r["normals"].reshape(-1, 3)[pick].double().cpu().numpy()
+    ids = hits.cpu().numpy()
+    d = depth[hits].double().cpu().numpy()
     P, D, t0, _ = otrace.rays(cams, width, height, ids=ids)
     X = (P - t0[:, None] * D) + d[:, None] * D
-    f = np.zeros(len(ids))
-    g = np.zeros((len(ids), 3))
-    for m in levels:
-        fl, gl = osiren.forward_grad(osiren.layers_from_model(m), X)
-        f += fl.reshape(-1)
-        g += gl
-    cosang = (nrm * g).sum(1) / (np.linalg.norm(nrm, axis=1) * np.linalg.norm(g, axis=1))
-    ang = np.degrees(np.arccos(np.clip(cosang, -1.0, 1.0)))
     x32 = torch.from_numpy(X.astype(np.float32)).cuda()
-    fp = stack.eval(x32).double().cpu().numpy()
-    f64 = np.zeros(len(ids))
-    for m in levels:
-        f64 += osiren.forward(osiren.layers_from_model(m), x32.double().cpu().numpy())
-    ev = np.abs(fp - f64)
-    return {"hit_sdf_abs_max": float(np.abs(f).max()), "hit_sdf_abs_mean": float(np.abs(f).mean()),
+    nrm = r["normals"].reshape(-1, 3)[hits].double()
+    tl = [[(torch.from_numpy(w).cuda(), torch.from_numpy(b).cuda()) for w, b in osiren.layers_from_model(m)]
+          for m in levels]
+    f_abs, ev, ang = [], [], []
+    for k0 in range(0, len(ids), chunk):
+        xs = x32[k0:k0 + chunk]
+        f64 = torch.zeros(len(xs), dtype=torch.float64, device="cuda")
+        g64 = torch.zeros(len(xs), 3, dtype=torch.float64, device="cuda")
+        for lay in tl:
+            fl, gl = osiren.torch_forward_grad(lay, xs.double())
+            f64 += fl
+            g64 += gl
+        fp = stack.eval(xs).double()
+        n = nrm[k0:k0 + chunk]
+        cosang = (n * g64).sum(1) / (n.norm(dim=1) * g64.norm(dim=1))
+        f_abs.append(f64.abs())
+        ev.append((fp - f64).abs())
+        ang.append(torch.rad2deg(torch.arccos(cosang.clamp(-1.0, 1.0))))
+    f_abs, ev, ang = torch.cat(f_abs), torch.cat(ev), torch.cat(ang)
+    return {"hit_sdf_abs_max": float(f_abs.max()), "hit_sdf_abs_mean": float(f_abs.mean()),
             "eval_err_max": float(ev.max()), "eval_err_mean": float(ev.mean()),
             "normal_deg_max": float(ang.max()), "normal_deg_mean": float(ang.mean()), "hits_checked": len(ids)}
 
@@ -279,7 +285,7 @@ def bench_trace(levels, precisions, reps=2, check=False):
     """16 seeded orbit cameras at 1920x1080 (one persistent launch + normals +
     shading per frame set), through the coarse model alone and through the
     three-level stack (bands 0.05, 0.02): rays per second; with check (CPU
-    leg) the FP64 errors at the hits of the last frame set."""
+    leg) the FP64 errors at every hit of the last frame set."""
     cam_list = orbit_cameras(16, seed=MASTER_SEED + 7)
     cams = camera_block(cam_list)
     out = {}
@@ -536,7 +542,7 @@ def main():
     if args.only:
         precs = args.precisions.split(",") if args.precisions != "bf16,bf16x3,fp16,fp32" else None
         fig = {"extract": lambda: bench_extract(levels, precs or ["bf16x3"], args.steps, args.warmup),
-               "trace": lambda: bench_trace(levels, precs or ["bf16", "fp32"], check=not args.no_cpu_baseline),
+               "trace": lambda: bench_trace(levels, precs or ["bf16", "bf16x3", "fp32"], check=not args.no_cpu_baseline),
                "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps),
                "train_data": lambda: bench_train_data(levels),
                "render": lambda: bench_render(tuple(precs or ("fp16", "fp32")), frames=args.steps * 20),
@@ -567,7 +573,7 @@ def main():
     else:
         head = "bf16x3" if "bf16x3" in results else next(iter(results))
 
-    tr = bench_trace(levels, ["bf16", "fp16", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
+    tr = bench_trace(levels, ["bf16", "fp16", "bf16x3", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
     if tr:
         print(f"# trace: {tr}", file=sys.stderr, flush=True)
     tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None

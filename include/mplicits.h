@@ -42,9 +42,10 @@ extern "C" {
 /* three-term BF16: every weight and activation as a BF16 pair hi + lo, each
  * product as hi*hi + hi*lo + lo*hi on the BF16 matrix cores with FP32
  * accumulation (about 16 significant bits per operand).  Levels wider than 256
- * run FP16 (their operand pairs would not fit a wave's registers).
- * Evaluation (mp_eval without gradients) and extraction only: the gradient,
- * tracing, rendering and batched entry points return MP_EUNSUPPORTED. */
+ * run FP16 (their operand pairs would not fit a wave's registers).  Served by
+ * evaluation (gradients in forward mode), extraction, tracing and rendering;
+ * the batched extraction and the training entry points return
+ * MP_EUNSUPPORTED. */
 #define MP_BF16X3 3
 
 /* sample coordinates of the extraction grid */

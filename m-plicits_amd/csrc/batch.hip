@@ -244,7 +244,7 @@ int mp_extract_batch_plan(const mp_extract_job* jobs, int n_jobs, void* stream, 
   hipStream_t s = as_stream(stream);
   const int shape = jobs[0].stack ? jobs[0].stack->shape : -1;
   MP_REQUIRE(!jobs[0].stack || jobs[0].stack->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_extract_batch_plan: %s",
-             "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
+             "three-term BF16 (MP_BF16X3) is not served by the batched extraction");
   int max_levels = 0;
   for (int j = 0; j < n_jobs; ++j) {
     const mp_extract_job& J = jobs[j];

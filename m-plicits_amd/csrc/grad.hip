@@ -267,18 +267,22 @@ static int launch_grad_width(const LevelDev& L, const float* xyz, int64_t n, flo
 int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n, float* f, float* g,
                       int accumulate, hipStream_t stream, const unsigned long long* n_dev) {
   const LevelDev& L = st->lv[level].dev;
+  // the level's own shape: a three-term BF16 stack runs its 512-wide levels in FP16
+  const int shape = st->lv[level].shape;
   // forward mode where the reverse-mode kernel spills: 16-bit width 512 (and
-  // 256 with one hidden GEMM), FP32 widths >= 256
+  // 256 with one hidden GEMM), FP32 widths >= 256, and every three-term BF16
+  // level (its operand pairs fill the registers the reverse mode would need)
   const int gemms = L.n_hidden - 1;
-  const bool fp32 = st->shape == SH_16x16x4_F32;
-  if (gemms >= 1 && (fp32 ? L.width >= 256 : (L.width == 512 || (L.width == 256 && gemms == 1))))
-    return launch_level_tangent(st->shape, L, xyz, n, f, g, accumulate, stream, n_dev);
-  switch (st->shape) {
+  const bool fp32 = shape == SH_16x16x4_F32;
+  if (shape == SH_32x32x16_BF16X3 ||
+      (gemms >= 1 && (fp32 ? L.width >= 256 : (L.width == 512 || (L.width == 256 && gemms == 1)))))
+    return launch_level_tangent(shape, L, xyz, n, f, g, accumulate, stream, n_dev);
+  switch (shape) {
     case SH_32x32x16_BF16: return launch_grad_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, n_dev, stream);
     case SH_32x32x16_F16: return launch_grad_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, n_dev, stream);
     case SH_16x16x4_F32: return launch_grad_width<S16x16x4_f32>(L, xyz, n, f, g, accumulate, n_dev, stream);
   }
-  set_error("gradient: unsupported fragment shape %d", st->shape);
+  set_error("gradient: unsupported fragment shape %d", shape);
   return MP_EUNSUPPORTED;
 }
 

@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void k_level_tangent(LevelDev L, const float* 
 template <class S, int W>
 static int launch_tangent_typed(const LevelDev& L, const float* xyz, int64_t n, float* f, float* g, int acc,
                                 const unsigned long long* n_dev, hipStream_t stream) {
-  constexpr int NC = default_cols<S>(W);
+  // three-term BF16: one column block (the operand pairs fill the registers)
+  constexpr int NC = S::SPLIT ? 1 : default_cols<S>(W);
   typedef LevelCore<S, W, NC, true, 4, true> M;
   static_assert(M::LDS_BYTES <= 160 * 1024, "LDS budget of one workgroup");
   const int64_t per_block = (int64_t)M::TILE;  // four waves x TILE / 4 points
@@ -150,6 +151,14 @@ int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t
     case SH_32x32x16_BF16: return launch_tangent_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, n_dev, stream);
     case SH_32x32x16_F16: return launch_tangent_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, n_dev, stream);
     case SH_16x16x4_F32: return launch_tangent_width<S16x16x4_f32>(L, xyz, n, f, g, accumulate, n_dev, stream);
+    case SH_32x32x16_BF16X3:
+      switch (L.width) {
+        case 64: return launch_tangent_typed<S32x32x16_bf16x3, 64>(L, xyz, n, f, g, accumulate, n_dev, stream);
+        case 128: return launch_tangent_typed<S32x32x16_bf16x3, 128>(L, xyz, n, f, g, accumulate, n_dev, stream);
+        case 256: return launch_tangent_typed<S32x32x16_bf16x3, 256>(L, xyz, n, f, g, accumulate, n_dev, stream);
+      }
+      set_error("gradient launch: three-term BF16 supports padded widths up to 256, got %d", L.width);
+      return MP_EUNSUPPORTED;
   }
   set_error("gradient launch: unsupported fragment shape %d", shape);
   return MP_EUNSUPPORTED;

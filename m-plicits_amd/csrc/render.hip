@@ -247,7 +247,6 @@ int mp_render(mp_stack* st, const float* cams, int n_cams, const mp_render_param
   MP_REQUIRE(st != nullptr && cams != nullptr && prm != nullptr && rgba != nullptr, MP_EINVAL,
              "mp_render: NULL argument");
   MP_REQUIRE(n_cams >= 1 && prm->width >= 1 && prm->height >= 1, MP_EINVAL, "mp_render: empty image");
-  MP_REQUIRE(st->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_render: %s", "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
   MP_REQUIRE(st->n_levels <= 3, MP_EUNSUPPORTED, "mp_render: the renderer draws at most three levels, got %d",
              st->n_levels);
   for (int l = 0; l < 3; ++l)
@@ -309,7 +308,7 @@ int mp_render(mp_stack* st, const float* cams, int n_cams, const mp_render_param
 
   hipLaunchKernelGGL(k_render_reset, dim3(1), dim3(64), 0, s, ctr);
   MP_LAUNCHED("k_render_reset");
-  const int nc = (st->nc == 2 && maxw <= 256) ? 2 : 1;
+  const int nc = (st->nc == 2 && maxw <= 256 && st->shape != SH_32x32x16_BF16X3) ? 2 : 1;
   switch (st->shape) {
     case SH_32x32x16_BF16:
       rc = nc == 2 ? launch_render<S32x32x16_bf16, 2>(a, s) : launch_render<S32x32x16_bf16, 1>(a, s);
@@ -319,6 +318,9 @@ int mp_render(mp_stack* st, const float* cams, int n_cams, const mp_render_param
       break;
     case SH_16x16x4_F32:
       rc = launch_render<S16x16x4_f32, 1>(a, s);
+      break;
+    case SH_32x32x16_BF16X3:
+      rc = launch_render<S32x32x16_bf16x3, 1>(a, s);
       break;
     default:
       set_error("mp_render: unsupported fragment shape %d", st->shape);

@@ -289,9 +289,15 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const TraceArgs* __restrict_
   typename S::Frag* hand = reinterpret_cast<typename S::Frag*>(reinterpret_cast<char*>(lds_raw) +
                                                                (threadIdx.x >> 6) * TRACE_HAND_BYTES);
   for (int j = 0; j < n_jobs; ++j) {
-    // two-block tiles unless a level of the job is 512 wide (FP32: always one)
-    if (S::ROWS == 32 && jobs[j].cols == 2) trace_rays<S, 2>(jobs[j], hand);
-    else trace_rays<S, 1>(jobs[j], hand);
+    // two-block tiles unless a level of the job is 512 wide (FP32 and
+    // three-term BF16: always one)
+    if constexpr (S::ROWS == 32 && !S::SPLIT) {
+      if (jobs[j].cols == 2) {
+        trace_rays<S, 2>(jobs[j], hand);
+        continue;
+      }
+    }
+    trace_rays<S, 1>(jobs[j], hand);
   }
 }
 
@@ -455,7 +461,6 @@ extern "C" {
 int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params* prm, float* depth,
              float* normals, uint32_t* rgba, int32_t* steps, int64_t* stats, void* stream) {
   MP_REQUIRE(st != nullptr && cams != nullptr && prm != nullptr, MP_EINVAL, "mp_trace: NULL argument");
-  MP_REQUIRE(st->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_trace: %s", "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
   MP_REQUIRE(n_cams >= 1 && prm->width >= 1 && prm->height >= 1, MP_EINVAL, "mp_trace: empty image");
   MP_REQUIRE(depth != nullptr && steps != nullptr && rgba != nullptr, MP_EINVAL,
              "mp_trace: depth, steps and rgba are required");
@@ -511,8 +516,9 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   MP_HIP(hipGetDevice(&dev));
   MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   // two column blocks per wave unless the stack asks for one (512-wide levels
-  // then run one block at a time, only where a ray needs them)
-  const int nc = st->nc == 2 ? 2 : 1;
+  // then run one block at a time, only where a ray needs them); three-term
+  // BF16 tiles hold one
+  const int nc = st->nc == 2 && st->shape != SH_32x32x16_BF16X3 ? 2 : 1;
   // persistent grid: one workgroup of four waves per CU (the LDS handoff
   // regions take 128 KB of the CU's LDS)
   const int blocks = cus;
@@ -527,6 +533,8 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
                        : launch_trace<S32x32x16_f16, 1>(blocks, lds, s, ta);
       case SH_16x16x4_F32:
         return launch_trace<S16x16x4_f32, 1>(blocks, lds, s, ta);
+      case SH_32x32x16_BF16X3:
+        return launch_trace<S32x32x16_bf16x3, 1>(blocks, lds, s, ta);
       default:
         set_error("mp_trace: unsupported fragment shape %d", st->shape);
         return MP_EUNSUPPORTED;
@@ -605,7 +613,6 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
     const mp_trace_params* prm = &J.params;
     MP_REQUIRE(J.stack != nullptr && J.cam != nullptr, MP_EINVAL, "mp_trace_batch_plan: job %d: NULL stack or camera", j);
     MP_REQUIRE(J.stack->shape == shape, MP_EINVAL, "mp_trace_batch_plan: job %d: every stack must have the same precision", j);
-    MP_REQUIRE(J.stack->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_trace_batch_plan: %s", "three-term BF16 (MP_BF16X3) serves evaluation and extraction only");
     MP_REQUIRE(J.depth != nullptr && J.steps != nullptr && J.rgba != nullptr, MP_EINVAL,
                "mp_trace_batch_plan: job %d: depth, steps and rgba are required", j);
     MP_REQUIRE(prm->width >= 1 && prm->height >= 1 && prm->max_steps >= 1 && prm->hit_eps > 0.f &&
@@ -662,7 +669,7 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
       a.levels[l] = J.stack->lv[l].dev;
       jw = J.stack->lv[l].width > jw ? J.stack->lv[l].width : jw;
     }
-    a.cols = (J.stack->nc == 2 && jw <= 256) ? 2 : 1;
+    a.cols = (J.stack->nc == 2 && jw <= 256 && J.stack->shape != SH_32x32x16_BF16X3) ? 2 : 1;
     for (int i = 0; i < J.params.n_deltas; ++i) a.deltas[i] = J.params.deltas[i];
     unsigned long long* c = p->ctr + (size_t)j * TB_CTR;
     a.next_ray = &c[0];
@@ -711,6 +718,7 @@ int mp_trace_batch_run(mp_trace_batch* p, void* stream) {
     case SH_32x32x16_BF16: rc = launch_trace_jobs<S32x32x16_bf16>(p->blocks, lds, s, jobs, p->n_jobs); break;
     case SH_32x32x16_F16: rc = launch_trace_jobs<S32x32x16_f16>(p->blocks, lds, s, jobs, p->n_jobs); break;
     case SH_16x16x4_F32: rc = launch_trace_jobs<S16x16x4_f32>(p->blocks, lds, s, jobs, p->n_jobs); break;
+    case SH_32x32x16_BF16X3: rc = launch_trace_jobs<S32x32x16_bf16x3>(p->blocks, lds, s, jobs, p->n_jobs); break;
     default:
       set_error("mp_trace_batch_run: unsupported fragment shape %d", p->shape);
       return MP_EUNSUPPORTED;

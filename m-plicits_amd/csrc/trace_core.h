@@ -13,6 +13,18 @@ namespace mp {
 // fits this region.
 constexpr size_t TRACE_HAND_BYTES = 32 * 1024;
 
+// Shape of the 512-wide levels of an S stack: the same, except that a
+// three-term BF16 stack packs them in FP16 (their operand pairs would not fit
+// a wave's registers).
+template <class S>
+struct WideShape {
+  typedef S type;
+};
+template <>
+struct WideShape<S32x32x16_bf16x3> {
+  typedef S32x32x16_f16 type;
+};
+
 // f_level at the NC points of this lane; hand: the wave's handoff region.
 // A 512-wide level runs one column block at a time (its operands fill the
 // registers for one), and only the blocks whose bit is set in cols (a
@@ -20,13 +32,15 @@ constexpr size_t TRACE_HAND_BYTES = 32 * 1024;
 template <class S, int NC>
 __device__ __forceinline__ void eval_level_any(const LevelDev& L, const float (&x)[NC][3], float (&f)[NC],
                                       typename S::Frag* hand, unsigned cols = ~0u) {
+  static_assert(!S::SPLIT || NC == 1, "three-term BF16 tiles hold one column block");
+  typedef typename WideShape<S>::type SW;
   switch (L.width) {
     case 64: LevelCore<S, 64, NC, false>::eval(L, x, f, nullptr, hand); break;
     case 128: LevelCore<S, 128, NC, false>::eval(L, x, f, nullptr, hand); break;
     case 256: LevelCore<S, 256, NC, false>::eval(L, x, f, nullptr, hand); break;
     default:
       if constexpr (NC == 1) {
-        LevelCore<S, 512, 1, false>::eval(L, x, f, nullptr, hand);
+        LevelCore<SW, 512, 1, false>::eval(L, x, f, nullptr, reinterpret_cast<typename SW::Frag*>(hand));
       } else {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {

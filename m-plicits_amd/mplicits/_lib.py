@@ -20,7 +20,7 @@ MP_EUNSUPPORTED = -4
 MP_FP32 = 0
 MP_BF16 = 1
 MP_FP16 = 2
-MP_BF16X3 = 3  # three-term BF16 (hi.hi + hi.lo + lo.hi); evaluation and extraction only
+MP_BF16X3 = 3  # three-term BF16 (hi.hi + hi.lo + lo.hi); no batched extraction or training
 PRECISIONS = {"fp32": MP_FP32, "bf16": MP_BF16, "fp16": MP_FP16, "bf16x3": MP_BF16X3}
 
 MP_GRID_REFERENCE = 0

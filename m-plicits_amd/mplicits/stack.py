@@ -42,7 +42,8 @@ class LevelStack:
     precision: "fp32", "bf16", "fp16" or "bf16x3": arithmetic of the hidden
         GEMMs.  "bf16x3" holds every operand as a BF16 pair hi + lo and forms
         hi.hi + hi.lo + lo.hi (levels wider than 256 run FP16); it serves
-        evaluation without gradients and extraction only.
+        evaluation (gradients in forward mode), extraction, tracing and
+        rendering, not batched extraction or training.
     """
 
     def __init__(self, models, precision="fp32"):

@@ -78,6 +78,21 @@ def torch_forward(layers, x):
     return h @ w.T + b
 
 
+def torch_forward_grad(layers, x):
+    """forward_grad in torch (for FP64 evaluation on a device): (f [n], grad f
+    [n, 3]) by the same forward-mode chain rule; layers are torch tensors of
+    x's dtype and device."""
+    h = x
+    jac = None
+    for k, (w, b) in enumerate(layers[:-1]):
+        a = h @ w.T + b
+        c = torch.cos(a)[:, :, None]
+        jac = c * w[None, :, :] if k == 0 else c * torch.matmul(w, jac)
+        h = torch.sin(a)
+    w, b = layers[-1]
+    return (h @ w.T + b)[:, 0], torch.matmul(w, jac)[:, 0, :]
+
+
 def reference_init_draws(hidden, w0, ww, seed):
     """Restatement of the RNG call order of SIREN(3, 1, hidden, w0, ww)
     (i3d/model.py:87-105 then reset_weights 131-141): every nn.Linear is

@@ -37,13 +37,15 @@ def rays(cameras, width, height, bound_radius=1.2, pixel_center=True, flip_y=Fal
         ny = -ny
     O = np.empty((len(ids), 3))
     D = np.empty((len(ids), 3))
-    for k, c in enumerate(cam):
+    for c in np.unique(cam):
+        k = np.nonzero(cam == c)[0]
         iv, ip = (np.asarray(m, np.float64) for m in cameras[c])
-        eye = ip @ np.array([nx[k], ny[k], -1.0, 1.0])
-        world = iv[:3, :3] @ eye[:3] + iv[:3, 3]
+        clip = np.stack([nx[k], ny[k], -np.ones(len(k)), np.ones(len(k))], 1)
+        eye = clip @ ip.T
+        world = eye[:, :3] @ iv[:3, :3].T + iv[:3, 3]
         o = iv[:3, 3]
         d = world - o
-        O[k], D[k] = o, d / np.linalg.norm(d)
+        O[k], D[k] = o, d / np.linalg.norm(d, axis=1, keepdims=True)
     b = (O * D).sum(1)
     c = (O * O).sum(1) - bound_radius ** 2
     disc = b * b - c

@@ -66,12 +66,13 @@ def test_batch_rejects_mixed_precision():
         BatchedExtraction([a, c], [16, 16], [[], []])
 
 
-def test_trace_batch_matches_single_model_trace():
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+def test_trace_batch_matches_single_model_trace(precision):
     from mplicits.synthetic import orbit_cameras
     from mplicits.tracer import BatchedTrace, trace
     specs = [(64, 1, (40, 24), []), (256, 3, (33, 20), [0.05, 0.02]), (512, 2, (48, 27), [0.04]),
              (128, 2, (20, 31), [0.05]), (128, 4, (36, 22), [0.05, 0.03, 0.02])]
-    stacks = [LevelStack(_model(w, l, 300 + k), "bf16") for k, (w, l, _, _) in enumerate(specs)]
+    stacks = [LevelStack(_model(w, l, 300 + k), precision) for k, (w, l, _, _) in enumerate(specs)]
     cams = orbit_cameras(len(specs), seed=5, aspect=1.5)
     refs = []
     for st, cam, (_, _, (W, H), d) in zip(stacks, cams, specs):

@@ -38,6 +38,29 @@ def test_gradient_against_autograd_and_finite_differences():
     np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-6 * np.abs(g).max())
 
 
+def test_torch_forward_grad_matches_autograd():
+    # the FP64 device evaluation the benchmark's checker runs at traced hits:
+    # pinned to torch autograd of the plain forward, and a dropped cosine
+    # factor (plain W . J) fails it
+    m, layers = _layers(2, (32, 24, 32))
+    x = torch.rand(97, 3, dtype=torch.float64) * 2 - 1
+    tl = [(torch.from_numpy(w), torch.from_numpy(b)) for w, b in layers]
+    f, g = osiren.torch_forward_grad(tl, x)
+    xr = x.clone().requires_grad_(True)
+    y = osiren.torch_forward(tl, xr)
+    (ga,) = torch.autograd.grad(y.sum(), xr)
+    np.testing.assert_allclose(f.numpy(), y.detach()[:, 0].numpy(), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g.numpy(), ga.numpy(), rtol=1e-10, atol=1e-10)
+    # contrast: the chain without the hidden layers' cosine factors
+    h, jac = x, None
+    for k, (w, b) in enumerate(tl[:-1]):
+        a = h @ w.T + b
+        jac = torch.cos(a)[:, :, None] * w[None] if k == 0 else torch.matmul(w, jac)
+        h = torch.sin(a)
+    g_bad = torch.matmul(tl[-1][0], jac)[:, 0, :]
+    assert (g_bad - ga).abs().max().item() > 1e-3 * ga.abs().max().item()
+
+
 def test_init_follows_reference_rng_order_and_bounds():
     torch.manual_seed(123)
     m = SIREN(3, 1, [64, 64, 64], w0=30, ww=30)

@@ -54,6 +54,23 @@ def test_ray_generation_centre_and_corner_pixels():
     np.testing.assert_allclose(D0[0], np.array([-tn1, -tn1, -1.0]) / math.sqrt(1 + 2 * tn1 * tn1), atol=1e-12)
 
 
+def test_rays_of_several_cameras_in_any_id_order():
+    # ids of three cameras, shuffled: each ray equals the same pixel of its
+    # camera generated alone
+    cams = [_camera(2.5, 30.0), _camera(3.0, 45.0), _camera(2.2, 60.0)]
+    W, H = 7, 5
+    ids = np.random.default_rng(3).permutation(3 * W * H)
+    P, D, t0, inside = otrace.rays(cams, W, H, ids=ids)
+    for c, cam in enumerate(cams):
+        Pc, Dc, tc, ic = otrace.rays([cam], W, H)
+        k = np.nonzero(ids // (W * H) == c)[0]
+        pix = ids[k] % (W * H)
+        np.testing.assert_allclose(D[k], Dc[pix], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(P[k], Pc[pix], rtol=0, atol=1e-13)
+        np.testing.assert_allclose(t0[k], tc[pix], rtol=0, atol=1e-13)
+        assert np.array_equal(inside[k], ic[pix])
+
+
 def test_ray_that_misses_the_sphere_and_eye_inside():
     # a wide field of view: the corner rays pass outside the sphere
     P, D, t0, inside = otrace.rays([_camera(2.5, 120.0)], 3, 3)

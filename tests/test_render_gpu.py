@@ -82,6 +82,17 @@ def test_frame_matches_oracle_bf16():
     _compare(out, ref, mask_agree=0.9, pos_tol=5e-3, color_agree=0.6)
 
 
+def test_frame_matches_oracle_bf16x3():
+    # three-term BF16 evaluates within a few 1e-6 of FP64: the FP32 bounds hold
+    levels = _levels()
+    cams = orbit_cameras(1, seed=4)
+    opts = dict(iters=(20, 5, 5), delta=0.05, normal_lod=2)
+    out = render(LevelStack(levels, "bf16x3"), cams, W, H, **opts)
+    ref = orender.render_fixed([orender.siren_level(osiren.layers_from_model(m)) for m in levels], cams, W, H,
+                               **opts)
+    _compare(out, ref)
+
+
 def test_finer_levels_run_only_where_a_step_reads_them():
     levels = _levels()
     cams = orbit_cameras(1, seed=3)
