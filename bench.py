@@ -481,10 +481,52 @@ def train_batch(levels, n, seed):
     return x, shape.sdf(x).float(), shape.normals(x).float()
 
 
-def bench_train(levels, precisions, steps=4):
+def train_grad_errors(levels, model, x, f_t, n_t, precision, n=1 << 14):
+    """CPU leg (checker): the parameter gradients of one step on the first n
+    points of the batch against the oracle's FP64 autograd (double backward
+    through the input gradient), evaluated in float64 on the device, under the
+    reference's residual loss (loss_add_detail_fine: squared SDF, normal
+    cosine, squared eikonal, weights 3e3 / 1e2 / 5e1), whose seeds are smooth
+    in F (the L1 terms of the timed loss flip sign with any rounding of F near
+    its target): relative error ||g - g_64|| / ||g_64|| per parameter tensor
+    (the largest) and over all parameters."""
+    from oracle import losses as olosses
+    from oracle import siren as osiren
+    coarse, med, _ = levels
+    xs, fs, ns = x[:n].contiguous(), f_t[:n], n_t[:n]
+
+    def loss_fn(F, G):
+        cos = torch.nn.functional.cosine_similarity(ns, G, dim=-1)
+        return (3e3 * ((fs - F) ** 2).mean() + 1e2 * (1 - cos).mean() + 5e1 * ((G.norm(dim=-1) - 1) ** 2).mean())
+
+    for p in model.parameters():
+        p.grad = None
+    training.train_step(model, xs, loss_fn, frozen=LevelStack([coarse, med], precision), precision=precision)
+    dev = lambda m: [(torch.from_numpy(w).cuda(), torch.from_numpy(b).cuda())  # noqa: E731
+                     for w, b in osiren.layers_from_model(m)]
+    fl = [(w.requires_grad_(True), b.requires_grad_(True)) for w, b in dev(model)]
+    ref_loss = sum(olosses.loss_add_detail_fine(fl, dev(med), dev(coarse), xs.double(), fs.double()[:, None],
+                                                ns.double()).values())
+    ref = torch.autograd.grad(ref_loss, [p for wb in fl for p in wb])
+    oms = [float(seq[1].w0) for seq in model.net[:-1]] + [1.0]
+    worst, num, den = 0.0, 0.0, 0.0
+    for k, p in enumerate(model.parameters()):
+        r = ref[k] * oms[k // 2]  # d/dW = omega d/dW_baked
+        e = (p.grad.double() - r).norm()
+        worst = max(worst, float(e / r.norm()))
+        num += float(e) ** 2
+        den += float(r.norm()) ** 2
+    for p in model.parameters():
+        p.grad = None
+    return {"points": n, "loss": "loss_add_detail_fine", "param_grad_rel_err_max_tensor": worst,
+            "param_grad_rel_err_all": (num / den) ** 0.5}
+
+
+def bench_train(levels, precisions, steps=4, check=False):
     """One Adam step of the fine 512x3 residual over the frozen coarse + medium
     levels on 2^18 points (half uniform in [-1,1]^3, half in the fine band),
-    config loss (L1 SDF + eikonal + normal cosine): training points per second."""
+    config loss (L1 SDF + eikonal + normal cosine): training points per second;
+    with check (CPU leg) the parameter gradients' error against FP64."""
     coarse, med, fine = levels
     n = 2 ** 18
     x, f_t, n_t = train_batch(levels, n, MASTER_SEED)
@@ -507,6 +549,8 @@ def bench_train(levels, precisions, steps=4):
         dt = (time.perf_counter() - t0) / steps
         out[prec] = {"train_pts_per_s": round(n / dt, 1), "ms_per_step": round(dt * 1e3, 2),
                      "loss": round(float(loss), 5)}
+        if check:
+            out[prec]["vs_fp64"] = train_grad_errors(levels, model, x, f_t, n_t, prec)
     return out
 
 
@@ -543,7 +587,8 @@ def main():
         precs = args.precisions.split(",") if args.precisions != "bf16,bf16x3,fp16,fp32" else None
         fig = {"extract": lambda: bench_extract(levels, precs or ["bf16x3"], args.steps, args.warmup),
                "trace": lambda: bench_trace(levels, precs or ["bf16", "bf16x3", "fp32"], check=not args.no_cpu_baseline),
-               "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps),
+               "train": lambda: bench_train(levels, precs or ["bf16", "fp32"], steps=args.steps,
+                                            check=not args.no_cpu_baseline),
                "train_data": lambda: bench_train_data(levels),
                "render": lambda: bench_render(tuple(precs or ("fp16", "fp32")), frames=args.steps * 20),
                "batch": lambda: bench_batch()}[args.only]()
@@ -576,7 +621,7 @@ def main():
     tr = bench_trace(levels, ["bf16", "fp16", "bf16x3", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
     if tr:
         print(f"# trace: {tr}", file=sys.stderr, flush=True)
-    tn = bench_train(levels, ["bf16", "fp32"]) if not args.no_extras else None
+    tn = bench_train(levels, ["bf16", "fp16", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
     if tn:
         print(f"# train: {tn}", file=sys.stderr, flush=True)
     td = bench_train_data(levels) if not args.no_extras else None

@@ -57,7 +57,7 @@ struct TrainPlan {
   size_t x4 = 0, w0 = 0, wl = 0, sg = 0;
   size_t bh[MAX_LAYERS] = {}, wf[MAX_LAYERS] = {}, wt[MAX_LAYERS] = {};
   size_t D[MAX_LAYERS] = {}, X[MAX_LAYERS] = {}, Xb[MAX_LAYERS] = {};
-  size_t A[2] = {}, Ab[2] = {}, part_out = 0, part_col = 0, part_dw = 0, tmp = 0, red = 0;
+  size_t A[2] = {}, Ab[2] = {}, part_out = 0, part_col = 0, part_dw = 0, tmp = 0, red = 0, scl = 0;
   size_t total = 0;
   int dw_chunks[MAX_LAYERS] = {};
 };
@@ -137,6 +137,7 @@ int plan_training(const mp_stack* st, int level, int64_t n, TrainPlan& p) {
   p.part_dw = take(dw);
   p.tmp = take((size_t)TMP_FLOATS * 4);
   p.red = take(256 * 4);
+  p.scl = take(2 * 4);
   p.total = off;
   return MP_OK;
 }
@@ -195,6 +196,54 @@ __global__ void k_tw_seeds(const float* __restrict__ s, const float* __restrict_
     }
   }
   sg[p] = v;
+}
+
+// FP16 seed scaling.  The backward stores every adjoint in the operand type;
+// the seeds of a mean loss over n points are O(1/n), and in FP16 (smallest
+// normal 6.1e-5) such adjoints lose their precision or flush to zero.  The
+// seeds are therefore scaled by a power of two S that brings max |seed| into
+// [2^4, 2^5) (room for the adjoints' growth through the layers below FP16's
+// 65504), and the parameter gradients by 1/S at the end; both scalings are
+// exact.  Block maxima of |sg| into red[0..255]:
+__global__ __launch_bounds__(256) void k_tw_absmax(const float* __restrict__ src, int64_t n, float* __restrict__ out) {
+  __shared__ float red[256];
+  float m = 0.f;
+  const int64_t step = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += step) m = fmaxf(m, fabsf(src[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// scl = (S, 1 / S) from the 256 block maxima (S = 1 for all-zero seeds)
+__global__ __launch_bounds__(256) void k_tw_seed_scale(const float* __restrict__ blockmax, float* __restrict__ scl) {
+  __shared__ float red[256];
+  red[threadIdx.x] = blockmax[threadIdx.x];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float m = red[0];
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
+    int k = m > 0.f ? 5 - e : 0;
+    k = k < -100 ? -100 : (k > 100 ? 100 : k);
+    scl[0] = ldexpf(1.f, k);
+    scl[1] = ldexpf(1.f, -k);
+  }
+}
+
+// v[i] *= scl[which] (grid-stride)
+__global__ void k_tw_scale(float* __restrict__ v, int64_t n, const float* __restrict__ scl, int which) {
+  const float f = scl[which];
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) v[i] *= f;
 }
 
 // hidden layer j as the B operands of the training GEMMs, zero padded, in
@@ -491,6 +540,13 @@ int mp_train_bwd(mp_stack* st, int level, const float* x, int64_t n, const float
   float* dB = dparams + p.woff[H + 1];  // the biases
   hipLaunchKernelGGL(k_tw_seeds, dim3(blocks_of(p.np, 256)), dim3(256), 0, s, dL_df, dL_dgrad, n, at4(st, p.sg), p.np);
   MP_LAUNCHED("k_tw_seeds");
+  const bool scaled = p.dtype == 2;  // FP16 adjoints: seed scaling (see k_tw_absmax)
+  if (scaled) {
+    hipLaunchKernelGGL(k_tw_absmax, dim3(256), dim3(256), 0, s, atf(st, p.sg), 4 * p.np, atf(st, p.red));
+    hipLaunchKernelGGL(k_tw_seed_scale, dim3(1), dim3(256), 0, s, atf(st, p.red), atf(st, p.scl));
+    hipLaunchKernelGGL(k_tw_scale, dim3(256), dim3(256), 0, s, atf(st, p.sg), 4 * p.np, atf(st, p.scl), 0);
+    MP_LAUNCHED("k_tw_seed_scale");
+  }
   // b_out: the sum of the value seeds
   hipLaunchKernelGGL(k_tw_sum, dim3(256), dim3(256), 0, s, atf(st, p.sg), p.np, 4, atf(st, p.red));
   hipLaunchKernelGGL(k_tw_sum, dim3(1), dim3(256), 0, s, atf(st, p.red), (int64_t)256, 1, dB + p.boff[H]);
@@ -549,6 +605,10 @@ int mp_train_bwd(mp_stack* st, int level, const float* x, int64_t n, const float
       rc = reduce_parts(part, WR, C, tmp, 1, p.w[0], 0, 1, 3 * (int64_t)p.wp[0], dB, 0, s);
       if (rc != MP_OK) return rc;
     }
+  }
+  if (scaled) {
+    hipLaunchKernelGGL(k_tw_scale, dim3(256), dim3(256), 0, s, dparams, p.n_params, atf(st, p.scl), 1);
+    MP_LAUNCHED("k_tw_scale");
   }
   return MP_OK;
 }

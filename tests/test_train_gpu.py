@@ -159,6 +159,40 @@ def test_train_entry_points_against_fp64_autograd(hidden, precision, tol):
     assert torch.equal(dp, dp2)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp16", 2e-3), ("bf16", 2e-2), ("fp32", 1e-5)])
+def test_train_gradients_with_mean_loss_seeds(precision, tol):
+    """Seeds of a mean loss over many points are O(1/n) (here 2^-16 .. 2^-14):
+    FP16 adjoints would flush to zero or keep few bits without the backward's
+    seed scaling.  The parameter gradients must stay within the precision's
+    bound of FP64 autograd, and in FP16 (where the seeds are normalised by a
+    power of two) seeds x 2^5 give exactly 2^5 x the gradients."""
+    m = seeded_siren([256] * 3, 90.0, seed=71, last_scale=0.1).cuda()
+    st = LevelStack([m], precision)
+    st.refresh()
+    gen = torch.Generator().manual_seed(6)
+    n = 5003
+    x = (torch.rand(n, 3, generator=gen) * 2 - 1).cuda()
+    s = (torch.randn(n, generator=gen) * 2.0 ** -15).cuda()
+    g = (torch.randn(n, 3, generator=gen) * 2.0 ** -16).cuda()
+    _, _, dp = _abi_train(st, x, s, g, keep=1)
+    lay = [(torch.from_numpy(w).requires_grad_(True), torch.from_numpy(b).requires_grad_(True))
+           for w, b in osiren.layers_from_model(m)]
+    fr, gr = olosses.level_value_grad(lay, x.double().cpu(), True)
+    obj = (s.double().cpu()[:, None] * fr).sum() + (g.double().cpu() * gr).sum()
+    ref = torch.autograd.grad(obj, [w for w, _ in lay] + [b for _, b in lay])
+    assert torch.isfinite(dp).all()
+    off = 0
+    for r in ref:
+        got = dp[off:off + r.numel()].double().cpu().view_as(r)
+        off += r.numel()
+        err = ((got - r).abs().max() / r.abs().max()).item()
+        print(f"{precision} {tuple(r.shape)}: max rel err {err:.2e}")
+        assert err <= tol, (tuple(r.shape), err)
+    if precision == "fp16":
+        _, _, dp32 = _abi_train(st, x, s * 32, g * 32, keep=1)
+        assert torch.equal(dp32, dp * 32)
+
+
 def test_train_entry_points_check_their_arguments():
     from mplicits import _lib
     lib = _lib.load()
