@@ -411,17 +411,47 @@ def bench_batch(precision="bf16", reps=3, n_models=24):
     """Batched uneven workload: 24 seeded models (width 64-512, 1-3 levels,
     own bumpy-sphere target, grids 128^3-512^3, one 1280x720 camera each).
     Extraction and tracing each run as one captured graph of persistent
-    launches (timed over graph replays); marching cubes per model, eagerly."""
+    launches (timed over graph replays), first with the default tile shapes,
+    then with the shapes the autotuner measures for each model's widest level
+    (batched-extraction and tracing families); marching cubes per model,
+    eagerly."""
+    from mplicits import autotune
     work = uneven_workload(n_models, device="cuda")
     stacks = [LevelStack(m["levels"], precision) for m in work]
-    ext = BatchedExtraction(stacks, [m["N"] for m in work], [m["deltas"] for m in work], chunk=CHUNK)
-    ext.run(graph=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
+
+    def time_extraction():
+        ext = BatchedExtraction(stacks, [m["N"] for m in work], [m["deltas"] for m in work], chunk=CHUNK)
         ext.run(graph=True)
-    torch.cuda.synchronize()
-    ext_ms = (time.perf_counter() - t0) / reps * 1e3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ext.run(graph=True)
+        torch.cuda.synchronize()
+        return ext, (time.perf_counter() - t0) / reps * 1e3
+
+    def time_trace():
+        tr = BatchedTrace([(st, m["camera"], 1280, 720, m["deltas"]) for st, m in zip(stacks, work)])
+        tr.run(graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            tr.run(graph=True)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        stats = tr.stats()
+        tr.close()
+        return ms, stats
+
+    ext, ext_ms_default = time_extraction()
+    ext.close()
+    del ext
+    tr_ms_default, _ = time_trace()
+    cache = osp.join("/tmp", f"mplicits_autotune_{os.getpid()}.txt")
+    tiles = {}
+    for st, m in zip(stacks, work):
+        got = autotune.tune_stack(st, cache=cache, families=("batch", "trace"))
+        tiles.setdefault(str(m["width"]), got)
+    ext, ext_ms = time_extraction()
     counts = ext.counts()
     flop = sum(c * level_flops(lv.hidden_widths()) for m, cs in zip(work, counts) for c, lv in zip(cs, m["levels"]))
     t0 = time.perf_counter()
@@ -433,16 +463,7 @@ def bench_batch(precision="bf16", reps=3, n_models=24):
     mc_ms = (time.perf_counter() - t0) * 1e3
     ext.close()
     del ext
-    tr = BatchedTrace([(st, m["camera"], 1280, 720, m["deltas"]) for st, m in zip(stacks, work)])
-    tr.run(graph=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        tr.run(graph=True)
-    torch.cuda.synchronize()
-    tr_ms = (time.perf_counter() - t0) / reps * 1e3
-    stats = tr.stats()
-    tr.close()
+    tr_ms, stats = time_trace()
     rays = n_models * 1280 * 720
     # algorithmic FLOP of the counted step evaluations (normals excluded)
     tr_flop = sum(e * level_flops(lv.hidden_widths()) for m, s in zip(work, stats)
@@ -454,31 +475,16 @@ def bench_batch(precision="bf16", reps=3, n_models=24):
             "levels": [len(m["levels"]) for m in work], "grids": [m["N"] for m in work],
             "grid_points": int(sum(m["N"] ** 3 for m in work)),
             "samples_per_level": [int(sum(c[l] for c in counts if l < len(c))) for l in range(3)],
-            "extract_ms": round(ext_ms, 2), "extract_tflops": round(flop / (ext_ms * 1e-3) / 1e12, 1),
+            "tiles_by_width": tiles,
+            "extract_ms": round(ext_ms, 2), "extract_ms_default_tiles": round(ext_ms_default, 2),
+            "extract_tflops": round(flop / (ext_ms * 1e-3) / 1e12, 1),
             "mc_ms": round(mc_ms, 2), "faces": faces,
-            "trace_ms": round(tr_ms, 2), "rays": rays, "rays_per_s": round(rays / (tr_ms * 1e-3), 1),
+            "trace_ms": round(tr_ms, 2), "trace_ms_default_tiles": round(tr_ms_default, 2),
+            "rays": rays, "rays_per_s": round(rays / (tr_ms * 1e-3), 1),
             "hits": int(sum(s["hits"] for s in stats)),
             "trace_evals_per_level": [int(sum(s["evals"][l] for s in stats if l < len(s["evals"]))) for l in range(3)],
             "trace_eval_tflops": round(tr_flop / (tr_ms * 1e-3) / 1e12, 1),
             "graphs": "extraction and tracing each captured once, timed over replays"}
-
-
-def train_batch(levels, n, seed):
-    """The training step's batch: n/2 points uniform in [-1, 1]^3 and n/2 in
-    the fine band |F_coarse + F_medium| < delta_2 (rejection sampling on the
-    frozen levels), with the analytic SDF and normals of the target."""
-    frozen = LevelStack(levels[:2], "fp32")
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    uni = torch.rand(n // 2, 3, device="cuda", generator=g) * 2 - 1
-    band, got = [], 0
-    while got < n - n // 2:
-        cand = torch.rand(1 << 20, 3, device="cuda", generator=g) * 2 - 1
-        keep = cand[frozen.eval(cand).abs() < DELTAS[1]]
-        band.append(keep)
-        got += len(keep)
-    x = torch.cat([uni, torch.cat(band)[:n - n // 2]]).contiguous()
-    shape = BumpySphere()
-    return x, shape.sdf(x).float(), shape.normals(x).float()
 
 
 def train_grad_errors(levels, model, x, f_t, n_t, precision, n=1 << 14):

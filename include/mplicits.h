@@ -91,13 +91,32 @@ int mp_stack_levels(const mp_stack* stack);
  * renderer/cuda/layer_gemms.cu:1612-1856). */
 int mp_stack_set_tiles(mp_stack* stack, int column_blocks);
 
-/* Autotuner of that knob: measures 1 and 2 column blocks for a seeded level
- * 3 -> width x n_hidden -> 1 in `precision` on a 160^3 grid (HIP events, after a
- * warm-up) and writes the faster one to *column_blocks.  cache_path (may be
- * NULL) is a text file of "width n_hidden precision column_blocks" lines: a
- * cached entry is returned without measuring, a new one is appended.
- * Allocates device memory; host-sync point; serialised by a process-wide
- * mutex.  No counterpart in the reference. */
+/* Tile shapes per kernel family (no counterpart in the reference):
+ *   MP_TILES_EXTRACT  the level kernels of mp_eval / mp_extract_volume, as
+ *                     mp_stack_set_tiles (1: one column block per wave in
+ *                     eight-wave workgroups, two waves per SIMD; 2: default);
+ *   MP_TILES_BATCH    the batched extraction kernel (1: each width's default
+ *                     column blocks per wave, 2: half of them);
+ *   MP_TILES_TRACE    the tracers (0: follow MP_TILES_EXTRACT, 1 or 2 column
+ *                     blocks per wave where the level widths allow two).
+ * The tile shape changes the schedule, never a value. */
+#define MP_TILES_EXTRACT 0
+#define MP_TILES_BATCH 1
+#define MP_TILES_TRACE 2
+int mp_stack_set_kernel_tiles(mp_stack* stack, int family, int tiles);
+
+/* Autotuner of those knobs: measures every tile shape of `family` for a
+ * seeded level 3 -> width x n_hidden -> 1 in `precision` (HIP events after a
+ * warm-up) and writes the fastest to *tiles:
+ *   MP_TILES_EXTRACT  mp_extract_volume over a 160^3 grid (1 or 2);
+ *   MP_TILES_BATCH    a one-job batched extraction over a 160^3 grid (1 or 2);
+ *   MP_TILES_TRACE    mp_trace of a 512^2 frame, at most 32 steps (1 or 2),
+ *                     per level evaluation.
+ * cache_path (may be NULL) is a text file of "family width n_hidden precision
+ * tiles" lines: a cached entry is returned without measuring, a new one is
+ * appended.  Allocates device memory; host-sync point; serialised by a
+ * process-wide mutex.  mp_autotune is the MP_TILES_EXTRACT family. */
+int mp_autotune_kernel(const char* cache_path, int family, int width, int n_hidden, int precision, int* tiles);
 int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, int* column_blocks);
 
 /* Evaluate sum_{l=lvl0}^{lvl1-1} f_l(x) at n points xyz [n][3] into sdf [n].

@@ -1,13 +1,20 @@
 // Tile-shape autotuner of the level kernels.
 //
 // The reference fixes its CUTLASS tile shapes at compile time
-// (renderer/cuda/layer_gemms.cu:1612-1856).  Here the point-column blocks per
-// wave (the knob of mp_stack_set_tiles) are measured on the device for one
-// (width, depth, precision): a stack with seeded weights evaluates a grid of
-// 2^22 samples once per candidate, timed with HIP events after a warm-up, and
-// the fastest candidate is returned.  Results are cached in a text file (one
-// "width n_hidden precision column_blocks" line per entry, guarded by a
-// process-wide mutex); a cached entry is returned without measuring.
+// (renderer/cuda/layer_gemms.cu:1612-1856).  Here each kernel family's tile
+// knob (mp_stack_set_kernel_tiles) is measured on the device for one
+// (width, depth, precision): a stack with seeded weights runs the family's
+// entry point once per candidate after a warm-up, timed with HIP events, and
+// the fastest candidate is returned.
+//   extraction  mp_extract_volume over 160^3 samples: one column block per
+//               wave in eight-wave workgroups, or the default tiles;
+//   batched     a one-job batched extraction over 160^3 samples: the width's
+//               default column blocks, or half of them;
+//   tracing     mp_trace of a 512^2 frame, at most 32 steps: one or two
+//               column blocks per wave, per level evaluation.
+// Results are cached in a text file (one "family width n_hidden precision
+// tiles" line per entry, guarded by a process-wide mutex); a cached entry is
+// returned without measuring.
 #include <cmath>
 #include <cstdio>
 #include <mutex>
@@ -20,13 +27,13 @@ namespace mp {
 
 static std::mutex g_tune_mutex;
 
-static bool cache_lookup(const char* path, int width, int n_hidden, int precision, int* out) {
+static bool cache_lookup(const char* path, int family, int width, int n_hidden, int precision, int* out) {
   FILE* f = path ? std::fopen(path, "r") : nullptr;
   if (!f) return false;
-  int w, h, p, c;
+  int fa, w, h, p, c;
   bool found = false;
-  while (std::fscanf(f, "%d %d %d %d", &w, &h, &p, &c) == 4) {
-    if (w == width && h == n_hidden && p == precision) {
+  while (std::fscanf(f, "%d %d %d %d %d", &fa, &w, &h, &p, &c) == 5) {
+    if (fa == family && w == width && h == n_hidden && p == precision) {
       *out = c;
       found = true;  // the last entry wins
     }
@@ -35,12 +42,25 @@ static bool cache_lookup(const char* path, int width, int n_hidden, int precisio
   return found;
 }
 
-static void cache_store(const char* path, int width, int n_hidden, int precision, int cols) {
+static void cache_store(const char* path, int family, int width, int n_hidden, int precision, int tiles) {
   if (!path) return;
   FILE* f = std::fopen(path, "a");
   if (!f) return;
-  std::fprintf(f, "%d %d %d %d\n", width, n_hidden, precision, cols);
+  std::fprintf(f, "%d %d %d %d %d\n", family, width, n_hidden, precision, tiles);
   std::fclose(f);
+}
+
+// Camera block of mp_trace (inverse view, inverse projection, column-major):
+// the eye at (0, 0, 2.5) looking down -z, 45 degree field of view, square.
+static void tune_camera(float* cam) {
+  for (int i = 0; i < 32; ++i) cam[i] = 0.f;
+  cam[0] = cam[5] = cam[10] = cam[15] = 1.f;
+  cam[14] = 2.5f;
+  const float t = 0.41421356f;  // tan(22.5 deg)
+  cam[16 + 0] = t;
+  cam[16 + 5] = t;
+  cam[16 + 10] = 1.f;
+  cam[16 + 15] = 1.f;
 }
 
 // Deterministic weights in the SIREN init range (layer 0: U(-1/3, 1/3) * 30,
@@ -61,14 +81,16 @@ using namespace mp;
 
 extern "C" {
 
-int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, int* column_blocks) {
-  MP_REQUIRE(column_blocks != nullptr, MP_EINVAL, "mp_autotune: column_blocks is NULL");
+int mp_autotune_kernel(const char* cache_path, int family, int width, int n_hidden, int precision, int* tiles) {
+  MP_REQUIRE(tiles != nullptr, MP_EINVAL, "mp_autotune_kernel: tiles is NULL");
+  MP_REQUIRE(family == MP_TILES_EXTRACT || family == MP_TILES_BATCH || family == MP_TILES_TRACE, MP_EINVAL,
+             "mp_autotune_kernel: unknown kernel family %d", family);
   MP_REQUIRE(width >= 1 && width <= 512 && n_hidden >= 1 && n_hidden <= 16, MP_EINVAL,
-             "mp_autotune: width must be in [1, 512] and n_hidden in [1, 16]");
+             "mp_autotune_kernel: width must be in [1, 512] and n_hidden in [1, 16]");
   MP_REQUIRE(precision == MP_FP32 || precision == MP_BF16 || precision == MP_FP16, MP_EINVAL,
-             "mp_autotune: unknown precision %d", precision);
+             "mp_autotune_kernel: unknown precision %d", precision);
   std::lock_guard<std::mutex> lock(g_tune_mutex);
-  if (cache_lookup(cache_path, width, n_hidden, precision, column_blocks)) return MP_OK;
+  if (cache_lookup(cache_path, family, width, n_hidden, precision, tiles)) return MP_OK;
 
   // seeded level 3 -> width x n_hidden -> 1
   std::vector<int> dims(n_hidden + 2, width);
@@ -81,27 +103,39 @@ int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, 
   }
   std::vector<float> hw, hb;
   for (int k = 0; k <= n_hidden; ++k) {
-    const float sc = k == 0 ? 10.0f : std::sqrt(6.0f / (float)dims[k]);
+    float sc = k == 0 ? 10.0f : std::sqrt(6.0f / (float)dims[k]);
+    // tracing: a small output (|f| of a few 1e-2) makes every ray take short
+    // steps through the whole sphere, so the frame measures the steady state
+    // of full tiles rather than rays leaving after a step or two
+    if (family == MP_TILES_TRACE && k == n_hidden) sc *= 0.03f;
     std::vector<float> w = seeded((size_t)dims[k] * dims[k + 1], sc, 7u + k);
-    std::vector<float> b = seeded((size_t)dims[k + 1], 0.1f, 101u + k);
+    std::vector<float> b = seeded((size_t)dims[k + 1], family == MP_TILES_TRACE && k == n_hidden ? 0.f : 0.1f, 101u + k);
     hw.insert(hw.end(), w.begin(), w.end());
     hb.insert(hb.end(), b.begin(), b.end());
   }
-  const int64_t N = 160;  // 4,096,000 samples
-  float *dw = nullptr, *db = nullptr, *vol = nullptr;
+  const int64_t N = 160;         // 4,096,000 samples
+  const int FW = 512, FH = 512;  // tracing frame
+  float *dw = nullptr, *db = nullptr, *vol = nullptr, *cam = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   mp_stack* st = nullptr;
-  int rc = MP_OK, best = 2;
+  int rc = MP_OK, best = family == MP_TILES_TRACE ? 2 : (family == MP_TILES_BATCH ? 1 : 2);
   float best_ms = 1e30f;
   auto fail = [&](hipError_t e, const char* what) {
-    set_error("mp_autotune: %s: %s", what, hipGetErrorString(e));
+    set_error("mp_autotune_kernel: %s: %s", what, hipGetErrorString(e));
     rc = MP_EHIP;
   };
+  const size_t vol_bytes = family == MP_TILES_TRACE ? (size_t)FW * FH * 12 : (size_t)N * N * N * 4;
   hipError_t e = hipMalloc(&dw, nw * 4);
   if (e == hipSuccess) e = hipMalloc(&db, nb * 4);
-  if (e == hipSuccess) e = hipMalloc(&vol, (size_t)N * N * N * 4);
+  if (e == hipSuccess) e = hipMalloc(&vol, vol_bytes);
+  if (e == hipSuccess) e = hipMalloc(&cam, 32 * 4);
   if (e == hipSuccess) e = hipMemcpy(dw, hw.data(), nw * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(db, hb.data(), nb * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    float hc[32];
+    tune_camera(hc);
+    e = hipMemcpy(cam, hc, sizeof(hc), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e != hipSuccess) fail(e, "setup");
@@ -110,11 +144,49 @@ int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, 
     mp_level_desc d{3, 1, n_hidden, hidden.data(), dw, db};
     rc = mp_stack_create(&d, 1, precision, nullptr, &st);
   }
-  for (int cols = 1; rc == MP_OK && cols <= 2; ++cols) {
-    rc = mp_stack_set_tiles(st, cols);
+  // one run of the family's entry point with the current tile shape
+  mp_batch* plan = nullptr;
+  mp_trace_params tp{};
+  tp.width = FW;
+  tp.height = FH;
+  tp.max_steps = 32;
+  tp.hit_eps = 1e-4f;
+  tp.bound_radius = 1.2f;
+  tp.band_scale = 1.4f;
+  tp.pixel_center = 1;
+  int64_t stats[MP_MAX_LEVELS + 1] = {0};
+  auto run = [&]() -> int {
+    switch (family) {
+      case MP_TILES_EXTRACT:
+        return mp_extract_volume(st, (int)N, MP_GRID_LATTICE, 32768, nullptr, 0, vol, nullptr, nullptr);
+      case MP_TILES_BATCH:
+        return mp_extract_batch_run(plan, nullptr);
+      default: {
+        float* depth = vol;
+        uint32_t* rgba = reinterpret_cast<uint32_t*>(vol + (size_t)FW * FH);
+        int32_t* steps = reinterpret_cast<int32_t*>(vol + 2 * (size_t)FW * FH);
+        return mp_trace(st, cam, 1, &tp, depth, nullptr, rgba, steps, stats, nullptr);
+      }
+    }
+  };
+  const int cands[2] = {1, 2};
+  for (int ci = 0; rc == MP_OK && ci < 2; ++ci) {
+    const int cand = cands[ci];
+    rc = mp_stack_set_kernel_tiles(st, family, cand);
+    if (rc == MP_OK && family == MP_TILES_BATCH) {
+      if (plan) mp_extract_batch_destroy(plan);
+      plan = nullptr;
+      mp_extract_job job{};
+      job.stack = st;
+      job.N = (int)N;
+      job.grid_mode = MP_GRID_LATTICE;
+      job.chunk = 32768;
+      job.vol = vol;
+      rc = mp_extract_batch_plan(&job, 1, nullptr, &plan);
+    }
     for (int rep = 0; rc == MP_OK && rep < 3; ++rep) {
       if (rep == 1) e = hipEventRecord(e0, nullptr);
-      if (rc == MP_OK) rc = mp_extract_volume(st, (int)N, MP_GRID_LATTICE, 32768, nullptr, 0, vol, nullptr, nullptr);
+      if (rc == MP_OK) rc = run();
     }
     if (rc != MP_OK) break;
     e = hipEventRecord(e1, nullptr);
@@ -125,21 +197,30 @@ int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, 
       fail(e, "timing");
       break;
     }
+    // tracing: per level evaluation (the two shapes may step rays differently
+    // only through the tail of the queue; the count normalises it)
+    if (family == MP_TILES_TRACE && stats[0] > 0) ms /= (float)stats[0];
     if (ms < best_ms) {
       best_ms = ms;
-      best = cols;
+      best = cand;
     }
   }
+  if (plan) mp_extract_batch_destroy(plan);
   if (st) mp_stack_destroy(st);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
+  if (cam) (void)hipFree(cam);
   if (vol) (void)hipFree(vol);
   if (db) (void)hipFree(db);
   if (dw) (void)hipFree(dw);
   if (rc != MP_OK) return rc;
-  cache_store(cache_path, width, n_hidden, precision, best);
-  *column_blocks = best;
+  cache_store(cache_path, family, width, n_hidden, precision, best);
+  *tiles = best;
   return MP_OK;
+}
+
+int mp_autotune(const char* cache_path, int width, int n_hidden, int precision, int* column_blocks) {
+  return mp_autotune_kernel(cache_path, MP_TILES_EXTRACT, width, n_hidden, precision, column_blocks);
 }
 
 }  // extern "C"

@@ -25,11 +25,12 @@ struct mp_batch {
   int blocks = 0;                // persistent workgroups per stage
   void* mem = nullptr;           // tables, counters, flags, lists (one allocation)
   size_t mem_bytes = 0;
-  void* level_tab[MP_MAX_LEVELS] = {};  // LevelJob [n]
-  int n_level_jobs[MP_MAX_LEVELS] = {};
+  // LevelJob tables per stage, one per tile shape (0: default, 1: half column blocks)
+  void* level_tab[MP_MAX_LEVELS][2] = {};
+  int n_level_jobs[MP_MAX_LEVELS][2] = {};
   void* band_tab[MP_MAX_LEVELS] = {};   // BandJob [n]
   int n_band_jobs[MP_MAX_LEVELS] = {};
-  uint32_t* counters = nullptr;  // [MP_MAX_LEVELS] stage item counters, then [n_jobs][MP_MAX_LEVELS] counts
+  uint32_t* counters = nullptr;  // [MP_MAX_LEVELS][2] stage item counters, then [n_jobs][MP_MAX_LEVELS] counts
   int64_t n_counters = 0;
   uint8_t* flags = nullptr;      // chunk flags of every model
   int64_t n_flags = 0;
@@ -66,25 +67,35 @@ __global__ void k_fill_bytes(uint8_t* p, int64_t n, uint8_t v) {
     p[i] = v;
 }
 
-template <class S, int W>
+// Column blocks per wave of the batched level kernel: the width's default
+// (the B operand set fills 128 VGPRs) or, in the HALF kernel, half of it (a
+// tile shape the autotuner may pick: half the registers and LDS hand-off per
+// wave).  The two shapes are separate kernels, each with one instantiation
+// per width.
+template <class S>
+constexpr int batch_cols(int W, bool half) {
+  return half && default_cols<S>(W) > 1 ? default_cols<S>(W) / 2 : default_cols<S>(W);
+}
+
+template <class S, int W, bool HALF>
 __device__ inline int64_t job_blocks(const EvalTarget& t) {
-  constexpr int per_block = 4 * LevelCore<S, W, default_cols<S>(W), true>::TILE;
+  constexpr int per_block = 4 * LevelCore<S, W, batch_cols<S>(W, HALF), true>::TILE;
   const int64_t n = (t.src == SRC_LIST) ? (int64_t)*t.list_count : t.n;
   return (n + per_block - 1) / per_block;
 }
 
-template <class S>
+template <class S, bool HALF>
 __device__ inline int64_t blocks_of(const LevelJob& j) {
   switch (j.L.width) {
-    case 64: return job_blocks<S, 64>(j.t);
-    case 128: return job_blocks<S, 128>(j.t);
-    case 256: return job_blocks<S, 256>(j.t);
-    default: return job_blocks<S, 512>(j.t);
+    case 64: return job_blocks<S, 64, HALF>(j.t);
+    case 128: return job_blocks<S, 128, HALF>(j.t);
+    case 256: return job_blocks<S, 256, HALF>(j.t);
+    default: return job_blocks<S, 512, HALF>(j.t);
   }
 }
 
 // Persistent level stage: every workgroup loops over blocks of the stage's jobs.
-template <class S>
+template <class S, bool HALF>
 __global__ __launch_bounds__(256) void k_level_jobs(const LevelJob* __restrict__ jobs, int n_jobs,
                                                     uint32_t* item_ctr) {
   extern __shared__ uint4 lds_raw[];
@@ -100,7 +111,7 @@ __global__ __launch_bounds__(256) void k_level_jobs(const LevelJob* __restrict__
     __syncthreads();  // every wave has read it before the LDS is reused
     int j = 0;
     for (; j < n_jobs; ++j) {
-      const int64_t nb = blocks_of<S>(jobs[j]);
+      const int64_t nb = blocks_of<S, HALF>(jobs[j]);
       if (b < nb) break;
       b -= nb;
     }
@@ -109,22 +120,22 @@ __global__ __launch_bounds__(256) void k_level_jobs(const LevelJob* __restrict__
     const int64_t n = (job.t.src == SRC_LIST) ? (int64_t)*job.t.list_count : job.t.n;
     // one call site per width keeps one copy of each width's code
     switch (job.L.width) {
-      case 64: eval_block<S, 64, default_cols<S>(64)>(job.L, job.t, n, b, lds); break;
-      case 128: eval_block<S, 128, default_cols<S>(128)>(job.L, job.t, n, b, lds); break;
-      case 256: eval_block<S, 256, default_cols<S>(256)>(job.L, job.t, n, b, lds); break;
-      default: eval_block<S, 512, default_cols<S>(512)>(job.L, job.t, n, b, lds); break;
+      case 64: eval_block<S, 64, batch_cols<S>(64, HALF)>(job.L, job.t, n, b, lds); break;
+      case 128: eval_block<S, 128, batch_cols<S>(128, HALF)>(job.L, job.t, n, b, lds); break;
+      case 256: eval_block<S, 256, batch_cols<S>(256, HALF)>(job.L, job.t, n, b, lds); break;
+      default: eval_block<S, 512, batch_cols<S>(512, HALF)>(job.L, job.t, n, b, lds); break;
     }
     __syncthreads();  // the next block may reuse the LDS ring
   }
 }
 
-template <class S>
+template <class S, bool HALF>
 constexpr size_t jobs_lds_bytes() {
   size_t m = 0;
-  const size_t b[4] = {LevelCore<S, 64, default_cols<S>(64), true>::LDS_BYTES,
-                       LevelCore<S, 128, default_cols<S>(128), true>::LDS_BYTES,
-                       LevelCore<S, 256, default_cols<S>(256), true>::LDS_BYTES,
-                       LevelCore<S, 512, default_cols<S>(512), true>::LDS_BYTES};
+  const size_t b[4] = {LevelCore<S, 64, batch_cols<S>(64, HALF), true>::LDS_BYTES,
+                       LevelCore<S, 128, batch_cols<S>(128, HALF), true>::LDS_BYTES,
+                       LevelCore<S, 256, batch_cols<S>(256, HALF), true>::LDS_BYTES,
+                       LevelCore<S, 512, batch_cols<S>(512, HALF), true>::LDS_BYTES};
   for (size_t v : b) m = v > m ? v : m;
   return m;
 }
@@ -200,24 +211,32 @@ __global__ __launch_bounds__(256) void k_band_select_jobs(const BandJob* __restr
   }
 }
 
-template <class S>
+template <class S, bool HALF>
 static int run_level_stage(mp_batch* p, int stage, hipStream_t s) {
-  constexpr size_t lds = jobs_lds_bytes<S>();
+  if (p->n_level_jobs[stage][HALF] == 0) return MP_OK;
+  constexpr size_t lds = jobs_lds_bytes<S, HALF>();
   static_assert(lds <= 160 * 1024, "LDS budget of one workgroup");
   static std::atomic<uint64_t> attr_devs{0};
-  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_level_jobs<S>), (int)lds));
-  hipLaunchKernelGGL(k_level_jobs<S>, dim3(p->blocks), dim3(256), lds, s,
-                     reinterpret_cast<const LevelJob*>(p->level_tab[stage]), p->n_level_jobs[stage],
-                     &p->counters[stage]);
+  MP_HIP(set_lds_once(attr_devs, reinterpret_cast<const void*>(&k_level_jobs<S, HALF>), (int)lds));
+  hipLaunchKernelGGL((k_level_jobs<S, HALF>), dim3(p->blocks), dim3(256), lds, s,
+                     reinterpret_cast<const LevelJob*>(p->level_tab[stage][HALF]), p->n_level_jobs[stage][HALF],
+                     &p->counters[2 * stage + HALF]);
   MP_LAUNCHED("k_level_jobs");
   return MP_OK;
 }
 
+// Level `stage` of every job: the default tile shape's jobs, then the halved one's.
+template <class S>
+static int run_level_both(mp_batch* p, int stage, hipStream_t s) {
+  const int rc = run_level_stage<S, false>(p, stage, s);
+  return rc != MP_OK ? rc : run_level_stage<S, true>(p, stage, s);
+}
+
 static int run_level(mp_batch* p, int stage, hipStream_t s) {
   switch (p->shape) {
-    case SH_32x32x16_BF16: return run_level_stage<S32x32x16_bf16>(p, stage, s);
-    case SH_32x32x16_F16: return run_level_stage<S32x32x16_f16>(p, stage, s);
-    case SH_16x16x4_F32: return run_level_stage<S16x16x4_f32>(p, stage, s);
+    case SH_32x32x16_BF16: return run_level_both<S32x32x16_bf16>(p, stage, s);
+    case SH_32x32x16_F16: return run_level_both<S32x32x16_f16>(p, stage, s);
+    case SH_16x16x4_F32: return run_level_both<S16x16x4_f32>(p, stage, s);
   }
   set_error("mp_extract_batch_run: unsupported fragment shape %d", p->shape);
   return MP_EUNSUPPORTED;
@@ -267,13 +286,14 @@ int mp_extract_batch_plan(const mp_extract_job* jobs, int n_jobs, void* stream, 
   // device layout: tables | counters | flags | per job: candidates [N^3], list [N^3]
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off = 0;
-  size_t off_ltab[MP_MAX_LEVELS], off_btab[MP_MAX_LEVELS];
+  size_t off_ltab[MP_MAX_LEVELS][2], off_btab[MP_MAX_LEVELS];
   for (int l = 0; l < max_levels; ++l) {
-    off_ltab[l] = off; off += al(sizeof(LevelJob) * n_jobs);
+    off_ltab[l][0] = off; off += al(sizeof(LevelJob) * n_jobs);
+    off_ltab[l][1] = off; off += al(sizeof(LevelJob) * n_jobs);
     off_btab[l] = off; off += al(sizeof(BandJob) * n_jobs);
   }
   const size_t off_ctr = off;
-  p->n_counters = MP_MAX_LEVELS + (int64_t)n_jobs * MP_MAX_LEVELS;
+  p->n_counters = 2 * MP_MAX_LEVELS + (int64_t)n_jobs * MP_MAX_LEVELS;
   off += al(p->n_counters * 4);
   const size_t off_flags = off;
   std::vector<size_t> flag_off(n_jobs), cand_off(n_jobs), list_off(n_jobs);
@@ -302,9 +322,10 @@ int mp_extract_batch_plan(const mp_extract_job* jobs, int n_jobs, void* stream, 
   char* base = reinterpret_cast<char*>(p->mem);
   p->counters = reinterpret_cast<uint32_t*>(base + off_ctr);
   p->flags = reinterpret_cast<uint8_t*>(base + off_flags);
-  uint32_t* counts = p->counters + MP_MAX_LEVELS;  // [j][0] candidates, [j][l] level-l selection
+  uint32_t* counts = p->counters + 2 * MP_MAX_LEVELS;  // [j][0] candidates, [j][l] level-l selection
 
-  std::vector<std::vector<LevelJob>> ltab(max_levels);
+  std::vector<std::vector<LevelJob>> ltab[2] = {std::vector<std::vector<LevelJob>>(max_levels),
+                                               std::vector<std::vector<LevelJob>>(max_levels)};
   std::vector<std::vector<BandJob>> btab(max_levels);
   for (int j = 0; j < n_jobs; ++j) {
     const mp_extract_job& J = jobs[j];
@@ -354,18 +375,23 @@ int mp_extract_batch_plan(const mp_extract_job* jobs, int n_jobs, void* stream, 
         bj.lcount = &counts[j * MP_MAX_LEVELS + l];
         btab[l].push_back(bj);
       }
-      ltab[l].push_back(lj);
+      // the stack's tile shape for the batched kernel (1: default, 2: half)
+      ltab[J.stack->batch_tiles == 2 ? 1 : 0][l].push_back(lj);
     }
   }
   for (int l = 0; l < max_levels; ++l) {
-    p->level_tab[l] = base + off_ltab[l];
     p->band_tab[l] = base + off_btab[l];
-    p->n_level_jobs[l] = (int)ltab[l].size();
     p->n_band_jobs[l] = (int)btab[l].size();
+    e = hipSuccess;
+    for (int h = 0; h < 2; ++h) {
+      p->level_tab[l][h] = base + off_ltab[l][h];
+      p->n_level_jobs[l][h] = (int)ltab[h][l].size();
+      if (e == hipSuccess && !ltab[h][l].empty())
+        e = hipMemcpyAsync(p->level_tab[l][h], ltab[h][l].data(), sizeof(LevelJob) * ltab[h][l].size(),
+                           hipMemcpyHostToDevice, s);
+    }
     int rc = MP_OK;
-    if (!ltab[l].empty()) {
-      e = hipMemcpyAsync(p->level_tab[l], ltab[l].data(), sizeof(LevelJob) * ltab[l].size(),
-                         hipMemcpyHostToDevice, s);
+    if (p->n_level_jobs[l][0] + p->n_level_jobs[l][1] > 0) {
       if (e == hipSuccess && !btab[l].empty())
         e = hipMemcpyAsync(p->band_tab[l], btab[l].data(), sizeof(BandJob) * btab[l].size(),
                            hipMemcpyHostToDevice, s);
@@ -411,7 +437,7 @@ int mp_extract_batch_run(mp_batch* p, void* stream) {
                          reinterpret_cast<const BandJob*>(p->band_tab[l]), p->n_band_jobs[l]);
       MP_LAUNCHED("k_band_select_jobs");
     }
-    if (p->n_level_jobs[l] > 0) {
+    if (p->n_level_jobs[l][0] + p->n_level_jobs[l][1] > 0) {
       rc = run_level(p, l, s);
       if (rc != MP_OK) return rc;
     }
@@ -423,7 +449,7 @@ int mp_extract_batch_counts(const mp_batch* p, int64_t* counts, void* stream) {
   MP_REQUIRE(p != nullptr && counts != nullptr, MP_EINVAL, "mp_extract_batch_counts: NULL argument");
   hipStream_t s = as_stream(stream);
   std::vector<uint32_t> h((size_t)p->n_jobs * MP_MAX_LEVELS);
-  MP_HIP(hipMemcpyAsync(h.data(), p->counters + MP_MAX_LEVELS, h.size() * 4, hipMemcpyDeviceToHost, s));
+  MP_HIP(hipMemcpyAsync(h.data(), p->counters + 2 * MP_MAX_LEVELS, h.size() * 4, hipMemcpyDeviceToHost, s));
   MP_HIP(hipStreamSynchronize(s));
   for (int j = 0; j < p->n_jobs; ++j) {
     for (int l = 0; l < MP_MAX_LEVELS; ++l) {

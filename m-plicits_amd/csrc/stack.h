@@ -22,6 +22,11 @@ struct mp_stack {
   int precision = MP_FP32;
   int shape = mp::SH_16x16x4_F32;  // of every level, except MP_BF16X3 stacks (per level)
   int nc = 1;  // point column blocks per wave
+  // tile shapes of the other kernel families (mp_stack_set_kernel_tiles):
+  // batched extraction (1: each width's default column blocks, 2: half of
+  // them) and tracing (0: follow nc, 1 or 2: column blocks per wave)
+  int batch_tiles = 1;
+  int trace_tiles = 0;
   Level lv[MP_MAX_LEVELS];
   void* scratch = nullptr;  // extraction / tracing work lists, grown on demand
   size_t scratch_bytes = 0;

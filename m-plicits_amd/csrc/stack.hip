@@ -317,6 +317,26 @@ int mp_stack_set_tiles(mp_stack* st, int column_blocks) {
   return MP_OK;
 }
 
+int mp_stack_set_kernel_tiles(mp_stack* st, int family, int tiles) {
+  MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_stack_set_kernel_tiles: stack is NULL");
+  switch (family) {
+    case MP_TILES_EXTRACT:
+      return mp_stack_set_tiles(st, tiles);
+    case MP_TILES_BATCH:
+      MP_REQUIRE(tiles == 1 || tiles == 2, MP_EINVAL,
+                 "mp_stack_set_kernel_tiles: batched tiles must be 1 (default) or 2 (half), got %d", tiles);
+      st->batch_tiles = tiles;
+      return MP_OK;
+    case MP_TILES_TRACE:
+      MP_REQUIRE(tiles >= 0 && tiles <= 2, MP_EINVAL,
+                 "mp_stack_set_kernel_tiles: tracing tiles must be 0, 1 or 2, got %d", tiles);
+      st->trace_tiles = tiles;
+      return MP_OK;
+  }
+  set_error("mp_stack_set_kernel_tiles: unknown kernel family %d", family);
+  return MP_EINVAL;
+}
+
 int mp_stack_update(mp_stack* st, void* stream) {
   MP_REQUIRE(st != nullptr, MP_EINVAL, "mp_stack_update: stack is NULL");
   ++st->version;  // drops the activations kept for training

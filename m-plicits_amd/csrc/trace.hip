@@ -518,7 +518,8 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   // two column blocks per wave unless the stack asks for one (512-wide levels
   // then run one block at a time, only where a ray needs them); three-term
   // BF16 tiles hold one
-  const int nc = st->nc == 2 && st->shape != SH_32x32x16_BF16X3 ? 2 : 1;
+  const int want = st->trace_tiles ? st->trace_tiles : st->nc;
+  const int nc = want == 2 && st->shape != SH_32x32x16_BF16X3 ? 2 : 1;
   // persistent grid: one workgroup of four waves per CU (the LDS handoff
   // regions take 128 KB of the CU's LDS)
   const int blocks = cus;
@@ -669,7 +670,8 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
       a.levels[l] = J.stack->lv[l].dev;
       jw = J.stack->lv[l].width > jw ? J.stack->lv[l].width : jw;
     }
-    a.cols = (J.stack->nc == 2 && jw <= 256 && J.stack->shape != SH_32x32x16_BF16X3) ? 2 : 1;
+    const int want = J.stack->trace_tiles ? J.stack->trace_tiles : J.stack->nc;
+    a.cols = (want == 2 && jw <= 256 && J.stack->shape != SH_32x32x16_BF16X3) ? 2 : 1;
     for (int i = 0; i < J.params.n_deltas; ++i) a.deltas[i] = J.params.deltas[i];
     unsigned long long* c = p->ctr + (size_t)j * TB_CTR;
     a.next_ray = &c[0];

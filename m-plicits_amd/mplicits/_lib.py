@@ -29,6 +29,11 @@ GRID_MODES = {"reference": MP_GRID_REFERENCE, "lattice": MP_GRID_LATTICE}
 
 MP_MAX_LEVELS = 4
 
+MP_TILES_EXTRACT = 0
+MP_TILES_BATCH = 1
+MP_TILES_TRACE = 2
+TILE_FAMILIES = {"extract": MP_TILES_EXTRACT, "batch": MP_TILES_BATCH, "trace": MP_TILES_TRACE}
+
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_int64_p = ctypes.POINTER(ctypes.c_int64)
 
@@ -117,6 +122,9 @@ SIGNATURES = {
     "mp_stack_set_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mp_autotune": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int)]),
+    "mp_stack_set_kernel_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "mp_autotune_kernel": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]),
     "mp_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mp_extract_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,

@@ -94,9 +94,12 @@ class LevelStack:
     def n_levels(self):
         return len(self.models)
 
-    def set_tiles(self, column_blocks):
-        """Points per wave = column_blocks x the MFMA block (the autotuner's knob)."""
-        _lib.check(self._lib.mp_stack_set_tiles(self._handle, int(column_blocks)))
+    def set_tiles(self, column_blocks, family="extract"):
+        """Tile shape of one kernel family (the autotuner's knobs):
+        "extract": 1 (one column block per wave, eight-wave workgroups) or 2
+        (default); "batch": 1 (each width's default column blocks) or 2 (half
+        of them); "trace": 0 (follow "extract"), 1 or 2 column blocks."""
+        _lib.check(self._lib.mp_stack_set_kernel_tiles(self._handle, _lib.TILE_FAMILIES[family], int(column_blocks)))
 
     def _current_versions(self):
         # the frequencies are read from the modules each time: update_omegas()

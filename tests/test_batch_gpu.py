@@ -100,7 +100,7 @@ def test_autotune_caches_and_applies(tmp_path):
     cache = str(tmp_path / "tiles.txt")
     cols = autotune.best_tiles(256, 3, "bf16", cache=cache)
     assert cols in (1, 2)
-    assert open(cache).read().split() == ["256", "3", "1", str(cols)]
+    assert open(cache).read().split() == ["0", "256", "3", "1", str(cols)]
     assert autotune.best_tiles(256, 3, "bf16", cache=cache) == cols  # cached: no second line
     assert len(open(cache).read().splitlines()) == 1
     st = LevelStack(_model(256, 2, 9), "bf16")
@@ -108,3 +108,43 @@ def test_autotune_caches_and_applies(tmp_path):
     x = torch.rand(4096, 3, device="cuda") * 2 - 1
     ref = LevelStack(_model(256, 2, 9), "bf16").eval(x)
     assert torch.equal(st.eval(x), ref)  # the tile shape changes the schedule, not the values
+    # the other families: one measured line each, applied to the stack
+    got = autotune.tune_stack(st, cache=cache, families=("batch", "trace"))
+    assert got["batch"] in (1, 2) and got["trace"] in (1, 2)
+    lines = [ln.split() for ln in open(cache).read().splitlines()]
+    assert [ln[0] for ln in lines] == ["0", "1", "2"]
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+def test_batch_tile_shapes_change_no_value(precision):
+    """Every job of the uneven batch under the halved tile shape of the
+    batched kernel gives the volumes of the default shape, bit for bit; the
+    tracer's one- and two-block tiles give the same frames."""
+    from mplicits.synthetic import orbit_cameras
+    from mplicits.tracer import BatchedTrace
+    stacks = [LevelStack(_model(w, l, 400 + k), precision) for k, (w, l, _, _) in enumerate(JOBS)]
+    runs = []
+    for tiles in (1, 2):
+        for st in stacks:
+            st.set_tiles(tiles, family="batch")
+        b = BatchedExtraction(stacks, [N for _, _, N, _ in JOBS], [d for _, _, _, d in JOBS], chunk=4096)
+        b.run()
+        torch.cuda.synchronize()
+        runs.append(([b.volume(j).clone() for j in range(len(JOBS))], b.counts()))
+        b.close()
+    for j in range(len(JOBS)):
+        assert torch.equal(runs[0][0][j], runs[1][0][j]), j
+    assert runs[0][1] == runs[1][1]
+    cams = orbit_cameras(3, seed=8, aspect=1.5)
+    frames = []
+    for tiles in (1, 2):
+        for st in stacks[:3]:
+            st.set_tiles(tiles, family="trace")
+        bt = BatchedTrace([(st, cam, 36, 24, d) for st, cam, (_, _, _, d) in zip(stacks[:3], cams, JOBS[:3])],
+                          max_steps=64)
+        bt.run()
+        torch.cuda.synchronize()
+        frames.append([t.clone() for t in bt.depth])
+        bt.close()
+    for a, b in zip(*frames):
+        assert torch.equal(a, b)
