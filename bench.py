@@ -487,6 +487,24 @@ def bench_batch(precision="bf16", reps=3, n_models=24):
             "graphs": "extraction and tracing each captured once, timed over replays"}
 
 
+def train_batch(levels, n, seed):
+    """The training step's batch: n/2 points uniform in [-1, 1]^3 and n/2 in
+    the fine band |F_coarse + F_medium| < delta_2 (rejection sampling on the
+    frozen levels), with the analytic SDF and normals of the target."""
+    frozen = LevelStack(levels[:2], "fp32")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    uni = torch.rand(n // 2, 3, device="cuda", generator=g) * 2 - 1
+    band, got = [], 0
+    while got < n - n // 2:
+        cand = torch.rand(1 << 20, 3, device="cuda", generator=g) * 2 - 1
+        keep = cand[frozen.eval(cand).abs() < DELTAS[1]]
+        band.append(keep)
+        got += len(keep)
+    x = torch.cat([uni, torch.cat(band)[:n - n // 2]]).contiguous()
+    shape = BumpySphere()
+    return x, shape.sdf(x).float(), shape.normals(x).float()
+
+
 def train_grad_errors(levels, model, x, f_t, n_t, precision, n=1 << 14):
     """CPU leg (checker): the parameter gradients of one step on the first n
     points of the batch against the oracle's FP64 autograd (double backward
