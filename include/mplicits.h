@@ -363,8 +363,11 @@ int mp_neighborhood_samples(const float* vert, int64_t V, int cols, const int64_
  *   faces   [F][3] int32 vertex ids,
  *   normals [V][3] float32 unit -grad(volume) (may be NULL),
  *   values  [V] float32 local value range (may be NULL).
- * Vertex and face ids are deterministic (prefix sums over 64 x 4 x 32 bricks
- * of the grid in a fixed order, and over plane, row and column inside one).
+ * Vertex and face ids are deterministic: exclusive prefix sums over the
+ * active points (those owning vertices or triangles) taken in a fixed order,
+ * tiles of 64 x 8 x 16 points (x, y, z) in grid order, and inside a tile by
+ * plane, row, then column.  The triangle set equals scikit-image's; only the
+ * numbering differs.
  * verts and faces may be NULL only when the counts are zero.
  * `ws` is a device workspace of mp_mc_workspace_size(nz, ny, nx) bytes, kept
  * between the two calls. */
