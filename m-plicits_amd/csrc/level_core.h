@@ -67,7 +67,8 @@ struct LevelCore {
 
   static constexpr bool STAGED = ALLOW_STAGED && sizeof(Frag) == 16 && LSTEPS % 32 == 0;
   static constexpr int SF = 16;  // fragments per stage
-  static constexpr int D = 4;    // LDS read-ahead (steps)
+  // LDS read-ahead in fragments (three-term: four steps of hi and lo fragments)
+  static constexpr int D = SPLIT ? 8 : 4;
   static constexpr int RING_FRAGS = STAGED ? 2 * SF * 64 : 0;
   static constexpr int PF_REG = sizeof(Frag) == 16 ? 8 : 32;
   static constexpr int PF = STAGED ? D : (LSTEPS < PF_REG ? LSTEPS : PF_REG);
@@ -340,6 +341,10 @@ struct LevelCore {
           for (int i = 0; i < QPK; ++i) epi(rb - 1, (kb / KPQ) * QPK + i, acc[(rb - 1) & 1]);
         }
         asm volatile("" ::: "memory");
+        // staged stream: the LDS look-ahead reads stay in the step that issues
+        // them (the compiler otherwise sinks them to their MFMA and waits on
+        // each one; the empty asm does not pin LDS loads)
+        if constexpr (STAGED) __builtin_amdgcn_sched_barrier(0);
       }
       if (rb + 1 < RBS) {
         bias = bias_next;
