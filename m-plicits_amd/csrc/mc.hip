@@ -310,13 +310,23 @@ __device__ inline void mc_tile_origin(const McTiles& T, int64_t t, int& sx, int&
   z0 = (int)(r / T.nsy) * MT_Z;
 }
 
+// Workgroups are dealt to the eight XCDs in turn (workgroup w to XCD w % 8).
+// The logical block of workgroup w gives every XCD one contiguous range of
+// blocks, so that the halo rows and columns a tile shares with its neighbours
+// in x and y are fetched into that XCD's L2 once.
+__device__ inline int64_t mc_xcd_block(unsigned w, unsigned nw) {
+  const unsigned k = w % 8u, per = nw / 8u, rem = nw % 8u;
+  return (int64_t)k * per + (k < rem ? k : rem) + w / 8u;
+}
+
 // mask[(z * ny + y) * nsx + sx] = active bits of the 64 points of row (y, z)
 // in x-segment sx; tile_tot[2 t] = active points of tile t (pair for the scan).
 __global__ __launch_bounds__(MC_BLOCK) void k_mc_classify(McVol V, McTiles T, unsigned long long* __restrict__ mask,
                                                           uint32_t* __restrict__ tile_tot) {
   // the wave index through readfirstlane: the compiler then knows t (and
   // every tile quantity) to be wave-uniform and keeps them in scalar registers
-  const int64_t t = (int64_t)blockIdx.x * (MC_BLOCK / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t t = mc_xcd_block(blockIdx.x, gridDim.x) * (MC_BLOCK / 64) +
+                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (t >= T.n) return;  // wave-uniform
   const int lane = threadIdx.x & 63;
   int sx, y0, z0;
