@@ -47,6 +47,15 @@ extern "C" {
  * the batched extraction and the training entry points return
  * MP_EUNSUPPORTED. */
 #define MP_BF16X3 3
+/* three-term BF16 for the first of a level's two hidden products, and the
+ * last one on the FP16 matrix cores with FP16 weights against the
+ * activations as an FP16 pair (a*hi + a*lo: two MFMAs per step instead of
+ * three).  Applies to levels with exactly three hidden layers up to width 256
+ * (other levels as MP_BF16X3).  Values only: evaluation without gradients and
+ * extraction; gradients, tracing, rendering, the batched extraction and the
+ * training entry points return MP_EUNSUPPORTED.  The error of such a level
+ * stays near 5e-5 where MP_BF16X3's is near 3e-6. */
+#define MP_BF16X3F 4
 
 /* sample coordinates of the extraction grid */
 #define MP_GRID_REFERENCE 0 /* i3d/meshing.py:42-63 as torch evaluates it (float32 true division) */

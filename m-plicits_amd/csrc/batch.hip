@@ -262,8 +262,9 @@ int mp_extract_batch_plan(const mp_extract_job* jobs, int n_jobs, void* stream, 
   MP_REQUIRE(jobs != nullptr && n_jobs >= 1, MP_EINVAL, "mp_extract_batch_plan: no jobs");
   hipStream_t s = as_stream(stream);
   const int shape = jobs[0].stack ? jobs[0].stack->shape : -1;
-  MP_REQUIRE(!jobs[0].stack || jobs[0].stack->precision != MP_BF16X3, MP_EUNSUPPORTED, "mp_extract_batch_plan: %s",
-             "three-term BF16 (MP_BF16X3) is not served by the batched extraction");
+  MP_REQUIRE(!jobs[0].stack || (jobs[0].stack->precision != MP_BF16X3 && jobs[0].stack->precision != MP_BF16X3F),
+             MP_EUNSUPPORTED, "mp_extract_batch_plan: %s",
+             "three-term BF16 (MP_BF16X3, MP_BF16X3F) is not served by the batched extraction");
   int max_levels = 0;
   for (int j = 0; j < n_jobs; ++j) {
     const mp_extract_job& J = jobs[j];

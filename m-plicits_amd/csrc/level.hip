@@ -71,6 +71,20 @@ static int launch_split(const LevelDev& L, const EvalTarget& t, hipStream_t stre
   return MP_EUNSUPPORTED;
 }
 
+// Three-term BF16 with the last hidden product in FP16 (levels with two
+// hidden products, one column block per wave as above).
+static int launch_split_f(const LevelDev& L, const EvalTarget& t, hipStream_t stream) {
+  MP_REQUIRE(L.n_hidden == 3, MP_EUNSUPPORTED, "level launch: the mixed three-term shape needs 3 hidden layers, got %d",
+             L.n_hidden);
+  switch (L.width) {
+    case 64: return launch_typed<S32x32x16_bf16x3f, 64, 1>(L, t, stream);
+    case 128: return launch_typed<S32x32x16_bf16x3f, 128, 1>(L, t, stream);
+    case 256: return launch_typed<S32x32x16_bf16x3f, 256, 1>(L, t, stream);
+  }
+  set_error("level launch: the mixed three-term shape supports padded widths up to 256, got %d", L.width);
+  return MP_EUNSUPPORTED;
+}
+
 // f and the input gradient at n points, forward mode (LevelCore's tangent
 // columns: four columns per point), for the widths where the reverse-mode
 // gradient kernel does not fit its registers.  Lane group 0 stores: the value
@@ -159,6 +173,9 @@ int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t
       }
       set_error("gradient launch: three-term BF16 supports padded widths up to 256, got %d", L.width);
       return MP_EUNSUPPORTED;
+    case SH_32x32x16_BF16X3F:
+      set_error("gradient launch: MP_BF16X3F (FP16 last product) serves values only; use MP_BF16X3 for gradients");
+      return MP_EUNSUPPORTED;
   }
   set_error("gradient launch: unsupported fragment shape %d", shape);
   return MP_EUNSUPPORTED;
@@ -171,6 +188,7 @@ int launch_level(const mp_stack* st, int level, const EvalTarget& t, hipStream_t
     case SH_32x32x16_F16: return launch_width<S32x32x16_f16>(L, st->nc, t, stream);
     case SH_16x16x4_F32: return launch_width<S16x16x4_f32>(L, st->nc, t, stream);
     case SH_32x32x16_BF16X3: return launch_split(L, t, stream);
+    case SH_32x32x16_BF16X3F: return launch_split_f(L, t, stream);
   }
   set_error("level launch: unsupported fragment shape %d", st->lv[level].shape);
   return MP_EUNSUPPORTED;

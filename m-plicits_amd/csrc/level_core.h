@@ -29,6 +29,8 @@
 //    overlap as far as one wave's in-order issue allows.
 #pragma once
 
+#include <type_traits>
+
 #include "eval_dispatch.h"
 
 namespace mp {
@@ -48,6 +50,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 // S::SPLIT (three-term BF16): every B fragment has a lo partner (registers
 // b[c][LO + kb], hand-off region after the hi fragments) and every A step two
 // fragments in the stream (hi, then lo); a step issues hi.hi, hi.lo and lo.hi.
+// S::LAST_F16 (levels with two hidden products): the first product hands over
+// its activations as FP16 pairs, and the last product's stream holds one FP16
+// fragment per step; a step there issues a.hi and a.lo on the FP16 cores.
 template <class S, int W, int NC, bool ALLOW_STAGED, int WAVES = 4, bool TANGENT = false>
 struct LevelCore {
   typedef typename S::Frag Frag;
@@ -59,6 +64,7 @@ struct LevelCore {
   static constexpr int KBS = W / KB;             // K blocks per layer
   static constexpr int STEPS = RBS * KBS;        // MFMA steps per layer
   static constexpr bool SPLIT = S::SPLIT;
+  static constexpr bool LF = S::LAST_F16;
   static constexpr int BP = SPLIT ? 2 : 1;       // fragments per operand (hi, lo)
   static constexpr int LO = SPLIT ? KBS : 0;     // offset of the lo B fragments
   static constexpr int LSTEPS = STEPS * BP;      // A fragments per layer
@@ -198,6 +204,14 @@ struct LevelCore {
     S::set(hi, e, v);
     if constexpr (SPLIT) S::set(lo, e, v - (float)hi[e]);
   }
+  // The same as an FP16 pair (operands of a LAST_F16 level's last product).
+  static __device__ __forceinline__ void set2h(Frag& hi, Frag& lo, int e, float v) {
+    f16x8 h = __builtin_bit_cast(f16x8, hi), l = __builtin_bit_cast(f16x8, lo);
+    h[e] = (_Float16)v;
+    l[e] = (_Float16)(v - (float)h[e]);
+    hi = __builtin_bit_cast(Frag, h);
+    lo = __builtin_bit_cast(Frag, l);
+  }
 
   // Layer-0 operands of the FP32 matrix cores (16-bit shapes), loaded by the
   // caller ahead of its own set-up (point coordinates) so that their latency
@@ -304,7 +318,7 @@ struct LevelCore {
   // Hidden GEMM j over all row blocks with the epilogue of block rb - 1 issued
   // in slices between the MFMAs of block rb.  epi(rb, q, acc) retires
   // accumulator register q of row block rb.
-  template <class Epi>
+  template <bool LASTF = false, class Epi>
   static __device__ __forceinline__ void gemm(Stream& A, int j, const float* __restrict__ bc,
                                      const Frag (&b)[NC][BP * KBS], int g, Epi epi) {
     Acc acc[2][NC];
@@ -321,8 +335,18 @@ struct LevelCore {
       Acc bias_next;
 #pragma unroll
       for (int kb = 0; kb < KBS; ++kb) {
-        const Frag a = A.next(j, BP * (rb * KBS + kb));
-        if constexpr (SPLIT) {
+        const Frag a = A.next(j, (LASTF ? 1 : BP) * (rb * KBS + kb));
+        if constexpr (LASTF) {
+          // FP16 weights against the activations as an FP16 pair
+          const f16x8 af = __builtin_bit_cast(f16x8, a);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            acc[rb & 1][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af, __builtin_bit_cast(f16x8, b[c][kb]),
+                                                                    kb == 0 ? bias : acc[rb & 1][c], 0, 0, 0);
+            acc[rb & 1][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af, __builtin_bit_cast(f16x8, b[c][LO + kb]),
+                                                                    acc[rb & 1][c], 0, 0, 0);
+          }
+        } else if constexpr (SPLIT) {
           const Frag al = A.next(j, BP * (rb * KBS + kb) + 1);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
@@ -391,13 +415,15 @@ struct LevelCore {
     A.begin();
 
     // hidden layers that feed another hidden layer: B registers -> LDS -> B registers
-    for (int j = 0; j + 1 < n_gemm; ++j) {
+    auto hidden = [&](int j, auto f16_out) {
+      constexpr bool H16 = decltype(f16_out)::value;
       Frag st[NC][BP];
       gemm(A, j, L.biasc + (size_t)j * W, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
         const int t = rb * CREGS + q;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          set2(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
+          if constexpr (H16) set2h(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
+          else set2(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
           if (t % EPL == EPL - 1) {
             hand[(c * KBS + t / EPL) * 64 + lane] = st[c][0];
             if constexpr (SPLIT) hand[((NC + c) * KBS + t / EPL) * 64 + lane] = st[c][1];
@@ -411,13 +437,18 @@ struct LevelCore {
           b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
           if constexpr (SPLIT) b[c][LO + kb] = hand[((NC + c) * KBS + kb) * 64 + lane];
         }
+    };
+    if constexpr (LF) {
+      hidden(0, std::true_type());  // the stack gives LAST_F16 only to levels with n_gemm == 2
+    } else {
+      for (int j = 0; j + 1 < n_gemm; ++j) hidden(j, std::false_type());
     }
 
     // last hidden layer fused with the output layer (output weights of row
     // block rb + 1 are fetched while rb retires)
     Acc wl, wl_next;
     loadc(L.wlc + g * CREGS, wl);
-    gemm(A, n_gemm - 1, L.biasc + (size_t)(n_gemm - 1) * W, b, g,
+    gemm<LF>(A, n_gemm - 1, L.biasc + (size_t)(n_gemm - 1) * W, b, g,
          [&](int rb, int q, const Acc (&acc)[NC]) {
            if (q == 0 && rb + 1 < RBS) loadc(L.wlc + ((rb + 1) * GROUPS + g) * CREGS, wl_next);
 #pragma unroll

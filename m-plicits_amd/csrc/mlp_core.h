@@ -44,11 +44,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // lo = bf16(v - hi)) and form each product as hi.hi + hi.lo + lo.hi: three
 // MFMAs per step, about 16 significant bits per operand.
 enum ShapeId { SH_32x32x16_BF16 = 0, SH_32x32x16_F16 = 1, SH_16x16x32_BF16 = 2,
-               SH_16x16x32_F16 = 3, SH_16x16x4_F32 = 4, SH_32x32x2_F32 = 5, SH_32x32x16_BF16X3 = 6 };
+               SH_16x16x32_F16 = 3, SH_16x16x4_F32 = 4, SH_32x32x2_F32 = 5, SH_32x32x16_BF16X3 = 6,
+               SH_32x32x16_BF16X3F = 7 };
 
 struct S32x32x16_bf16 {
   static constexpr bool FIRST_IN_REVS = true;
-  static constexpr bool SPLIT = false;
+  static constexpr bool SPLIT = false, LAST_F16 = false;
   static constexpr int ROWS = 32, KB = 16, EPL = 8, CREGS = 16, GROUPS = 2;
   typedef bf16x8 Frag;
   typedef f32x16 Acc;
@@ -65,9 +66,18 @@ struct S32x32x16_bf16x3 : S32x32x16_bf16 {
   static constexpr bool SPLIT = true;
 };
 
+// Three-term BF16 for every hidden product but the last, which runs on the
+// FP16 matrix cores with single FP16 weights and the activations as an FP16
+// pair: a.(bh + bl), two MFMAs per step (levels with exactly two hidden
+// products).  The last product's error reaches the output through one sine
+// and the output weights only; the error of the level stays near 5e-5.
+struct S32x32x16_bf16x3f : S32x32x16_bf16x3 {
+  static constexpr bool LAST_F16 = true;
+};
+
 struct S32x32x16_f16 {
   static constexpr bool FIRST_IN_REVS = true;
-  static constexpr bool SPLIT = false;
+  static constexpr bool SPLIT = false, LAST_F16 = false;
   static constexpr int ROWS = 32, KB = 16, EPL = 8, CREGS = 16, GROUPS = 2;
   typedef f16x8 Frag;
   typedef f32x16 Acc;
@@ -80,7 +90,7 @@ struct S32x32x16_f16 {
 
 struct S16x16x32_bf16 {
   static constexpr bool FIRST_IN_REVS = true;
-  static constexpr bool SPLIT = false;
+  static constexpr bool SPLIT = false, LAST_F16 = false;
   static constexpr int ROWS = 16, KB = 32, EPL = 8, CREGS = 4, GROUPS = 4;
   typedef bf16x8 Frag;
   typedef f32x4 Acc;
@@ -93,7 +103,7 @@ struct S16x16x32_bf16 {
 
 struct S16x16x32_f16 {
   static constexpr bool FIRST_IN_REVS = true;
-  static constexpr bool SPLIT = false;
+  static constexpr bool SPLIT = false, LAST_F16 = false;
   static constexpr int ROWS = 16, KB = 32, EPL = 8, CREGS = 4, GROUPS = 4;
   typedef f16x8 Frag;
   typedef f32x4 Acc;
@@ -106,7 +116,7 @@ struct S16x16x32_f16 {
 
 struct S16x16x4_f32 {
   static constexpr bool FIRST_IN_REVS = false;
-  static constexpr bool SPLIT = false;
+  static constexpr bool SPLIT = false, LAST_F16 = false;
   static constexpr int ROWS = 16, KB = 4, EPL = 1, CREGS = 4, GROUPS = 4;
   typedef float Frag;
   typedef f32x4 Acc;
@@ -119,7 +129,7 @@ struct S16x16x4_f32 {
 
 struct S32x32x2_f32 {
   static constexpr bool FIRST_IN_REVS = false;
-  static constexpr bool SPLIT = false;
+  static constexpr bool SPLIT = false, LAST_F16 = false;
   static constexpr int ROWS = 32, KB = 2, EPL = 1, CREGS = 16, GROUPS = 2;
   typedef float Frag;
   typedef f32x16 Acc;
@@ -136,7 +146,8 @@ struct ShapeInfo {
 };
 __host__ __device__ inline ShapeInfo shape_info(int id) {
   switch (id) {
-    case SH_32x32x16_BF16: case SH_32x32x16_F16: case SH_32x32x16_BF16X3: return {32, 16, 8, 16, 2, 2};
+    case SH_32x32x16_BF16: case SH_32x32x16_F16: case SH_32x32x16_BF16X3: case SH_32x32x16_BF16X3F:
+      return {32, 16, 8, 16, 2, 2};
     case SH_16x16x32_BF16: case SH_16x16x32_F16: return {16, 32, 8, 4, 4, 2};
     case SH_16x16x4_F32: return {16, 4, 1, 4, 4, 4};
     default: return {32, 2, 1, 16, 2, 4};

@@ -258,6 +258,8 @@ int mp_render(mp_stack* st, const float* cams, int n_cams, const mp_render_param
   MP_REQUIRE(prm->shading == MP_SHADE_NORMALS || prm->shading == MP_SHADE_PHONG, MP_EINVAL,
              "mp_render: unknown shading %d", prm->shading);
   MP_REQUIRE(prm->bound_radius > 0.f && prm->threshold >= 0.f, MP_EINVAL, "mp_render: bad parameters");
+  MP_REQUIRE(st->precision != MP_BF16X3F, MP_EUNSUPPORTED,
+             "mp_render: MP_BF16X3F (FP16 last product) serves extraction and values only; use MP_BF16X3");
   hipStream_t s = as_stream(stream);
   const int64_t total = (int64_t)n_cams * prm->width * prm->height;
   MP_REQUIRE(total < (1ll << 32), MP_EUNSUPPORTED, "mp_render: too many rays");

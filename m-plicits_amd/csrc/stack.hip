@@ -143,7 +143,14 @@ __global__ void k_pack_hidden(PackArgs a, int j, int transpose) {
   else w = (u < out_j && r < in_j) ? a.W[a.woff[j] + (int64_t)u * in_j + r] : 0.f;
   const int64_t o = (int64_t)(j - 1) * per_layer + i;
   void* dst = transpose ? a.apkt : a.apk;
-  if (a.shape == SH_32x32x16_BF16X3 && !transpose) {
+  if (a.shape == SH_32x32x16_BF16X3F && !transpose && j == a.n_hidden - 1) {
+    // the last hidden product of the mixed shape: one FP16 fragment per step,
+    // at the layer's offset in the pair layout
+    const int64_t oh = ((int64_t)(j - 1) * per_layer * 2) + (blk * 64 + lane) * s.epl + e;
+    reinterpret_cast<_Float16*>(dst)[oh] = (_Float16)w;
+    return;
+  }
+  if ((a.shape == SH_32x32x16_BF16X3 || a.shape == SH_32x32x16_BF16X3F) && !transpose) {
     const __bf16 hi = (__bf16)w;
     const int64_t oh = ((int64_t)(j - 1) * per_layer * 2) + ((blk * 2) * 64 + lane) * s.epl + e;
     reinterpret_cast<__bf16*>(dst)[oh] = hi;
@@ -151,7 +158,7 @@ __global__ void k_pack_hidden(PackArgs a, int j, int transpose) {
     return;
   }
   switch (a.shape) {
-    case SH_32x32x16_BF16X3:
+    case SH_32x32x16_BF16X3: case SH_32x32x16_BF16X3F:
     case SH_32x32x16_BF16: case SH_16x16x32_BF16: reinterpret_cast<__bf16*>(dst)[o] = (__bf16)w; break;
     case SH_32x32x16_F16: case SH_16x16x32_F16: reinterpret_cast<_Float16*>(dst)[o] = (_Float16)w; break;
     default: reinterpret_cast<float*>(dst)[o] = w; break;
@@ -223,7 +230,8 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
   *out = nullptr;
   MP_REQUIRE(levels != nullptr && n_levels >= 1 && n_levels <= MP_MAX_LEVELS, MP_EINVAL,
              "mp_stack_create: n_levels must be in [1, %d], got %d", MP_MAX_LEVELS, n_levels);
-  MP_REQUIRE(precision == MP_FP32 || precision == MP_BF16 || precision == MP_FP16 || precision == MP_BF16X3,
+  MP_REQUIRE(precision == MP_FP32 || precision == MP_BF16 || precision == MP_FP16 || precision == MP_BF16X3 ||
+                 precision == MP_BF16X3F,
              MP_EINVAL, "mp_stack_create: unknown precision %d", precision);
   mp_stack* st = new mp_stack();
   st->n_levels = n_levels;
@@ -231,6 +239,7 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
   st->shape = precision == MP_BF16 ? SH_32x32x16_BF16
               : precision == MP_FP16 ? SH_32x32x16_F16
               : precision == MP_BF16X3 ? SH_32x32x16_BF16X3
+              : precision == MP_BF16X3F ? SH_32x32x16_BF16X3F
                                      : SH_16x16x4_F32;
   int maxw = 0;
   for (int l = 0; l < n_levels; ++l) {
@@ -262,10 +271,15 @@ int mp_stack_create(const mp_level_desc* levels, int n_levels, int precision, vo
     }
     maxw = L.width > maxw ? L.width : maxw;
     L.n_hidden = d.n_hidden;
-    // three-term BF16 up to width 256; wider levels FP16
-    L.shape = (st->shape == SH_32x32x16_BF16X3 && L.width > 256) ? SH_32x32x16_F16 : st->shape;
+    // three-term BF16 up to width 256; wider levels FP16; the mixed shape
+    // (FP16 last product) for levels with two hidden products, the others
+    // three-term throughout
+    const bool split = st->shape == SH_32x32x16_BF16X3 || st->shape == SH_32x32x16_BF16X3F;
+    L.shape = (split && L.width > 256) ? SH_32x32x16_F16
+              : (st->shape == SH_32x32x16_BF16X3F && d.n_hidden != 3) ? SH_32x32x16_BF16X3
+                                                                       : st->shape;
     const ShapeInfo si = shape_info(L.shape);
-    const size_t parts = L.shape == SH_32x32x16_BF16X3 ? 2 : 1;
+    const size_t parts = (L.shape == SH_32x32x16_BF16X3 || L.shape == SH_32x32x16_BF16X3F) ? 2 : 1;
     L.weights = d.weights;
     L.biases = d.biases;
     const size_t W = (size_t)L.width;

@@ -468,6 +468,8 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
              st->n_levels - 1, prm->n_deltas);
   MP_REQUIRE(prm->max_steps >= 1 && prm->hit_eps > 0.f && prm->bound_radius > 0.f, MP_EINVAL,
              "mp_trace: bad parameters");
+  MP_REQUIRE(st->precision != MP_BF16X3F, MP_EUNSUPPORTED,
+             "mp_trace: MP_BF16X3F (FP16 last product) serves extraction and values only; use MP_BF16X3");
   hipStream_t s = as_stream(stream);
   const int64_t total = (int64_t)n_cams * prm->width * prm->height;
   MP_REQUIRE(total < (1ll << 32), MP_EUNSUPPORTED, "mp_trace: too many rays");
@@ -613,6 +615,8 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
     const mp_trace_job& J = jobs[j];
     const mp_trace_params* prm = &J.params;
     MP_REQUIRE(J.stack != nullptr && J.cam != nullptr, MP_EINVAL, "mp_trace_batch_plan: job %d: NULL stack or camera", j);
+    MP_REQUIRE(J.stack->precision != MP_BF16X3F, MP_EUNSUPPORTED,
+               "mp_trace_batch_plan: job %d: MP_BF16X3F serves extraction and values only; use MP_BF16X3", j);
     MP_REQUIRE(J.stack->shape == shape, MP_EINVAL, "mp_trace_batch_plan: job %d: every stack must have the same precision", j);
     MP_REQUIRE(J.depth != nullptr && J.steps != nullptr && J.rgba != nullptr, MP_EINVAL,
                "mp_trace_batch_plan: job %d: depth, steps and rgba are required", j);

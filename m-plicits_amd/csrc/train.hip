@@ -69,8 +69,8 @@ int plan_training(const mp_stack* st, int level, int64_t n, TrainPlan& p) {
   MP_REQUIRE(level >= 0 && level < st->n_levels, MP_EINVAL, "training: level %d of a %d-level stack", level,
              st->n_levels);
   MP_REQUIRE(n >= 1, MP_EINVAL, "training: n must be >= 1, got %lld", (long long)n);
-  MP_REQUIRE(st->precision != MP_BF16X3, MP_EUNSUPPORTED,
-             "training: three-term BF16 (MP_BF16X3) is not served by the training entry points");
+  MP_REQUIRE(st->precision != MP_BF16X3 && st->precision != MP_BF16X3F, MP_EUNSUPPORTED,
+             "training: three-term BF16 (MP_BF16X3, MP_BF16X3F) is not served by the training entry points");
   const mp_stack::Level& L = st->lv[level];
   p.H = L.n_hidden;
   MP_REQUIRE(p.H >= 2 && p.H < MAX_LAYERS, MP_EUNSUPPORTED, "training: a level needs 2..%d sine layers, got %d",

@@ -39,11 +39,15 @@ class LevelStack:
     Parameters
     ----------
     models: sequence of SIREN modules (3 -> ... -> 1), one per level.
-    precision: "fp32", "bf16", "fp16" or "bf16x3": arithmetic of the hidden
-        GEMMs.  "bf16x3" holds every operand as a BF16 pair hi + lo and forms
-        hi.hi + hi.lo + lo.hi (levels wider than 256 run FP16); it serves
-        evaluation (gradients in forward mode), extraction, tracing and
-        rendering, not batched extraction or training.
+    precision: "fp32", "bf16", "fp16", "bf16x3" or "bf16x3f": arithmetic of
+        the hidden GEMMs.  "bf16x3" holds every operand as a BF16 pair hi + lo
+        and forms hi.hi + hi.lo + lo.hi (levels wider than 256 run FP16); it
+        serves evaluation (gradients in forward mode), extraction, tracing and
+        rendering, not batched extraction or training.  "bf16x3f" is the same
+        with the last of a three-hidden-layer level's two products on the FP16
+        matrix cores (FP16 weights against the activations as an FP16 pair:
+        two MFMAs per step instead of three, level error near 5e-5); it serves
+        values only (evaluation without gradients and extraction).
     """
 
     def __init__(self, models, precision="fp32"):

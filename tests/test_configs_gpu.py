@@ -36,7 +36,7 @@ COARSE = osp.join(ROOT, "tests", "golden", "bumpy_coarse_256x3.pth")
 CHUNK = 32 ** 3
 # max |SDF - FP64| stated per precision (samples whose band decision flips are
 # counted apart); the north star's bound for the mixed-precision path is 1e-4
-SDF_TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "fp16": 5e-4, "bf16": 3e-3}
+SDF_TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "bf16x3f": 1e-4, "fp16": 5e-4, "bf16": 3e-3}
 
 
 def _three_levels():
@@ -71,7 +71,7 @@ def test_config1_fitted_coarse_dense_grid_and_mesh():
     assert len(np.unique(fa)) - len(edges) + len(fa) == 2
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x3f", "fp16", "bf16"])
 def test_config2_three_level_band_culled_64(precision):
     levels = _three_levels()
     N = 64
@@ -83,7 +83,7 @@ def test_config2_three_level_band_culled_64(precision):
           f"counts {counts}")
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x3f"])
 def test_config2_three_level_band_culled_512_sampled(precision):
     levels = _three_levels()
     N = 512
@@ -118,7 +118,7 @@ def test_config2_mixed_precision_mesh_tolerances():
     N = 256
     ref = _mesh(LevelStack(levels, "fp32").extract_volume(N, list(DELTAS), chunk=CHUNK)[0], N)
     # stated tolerances: (chamfer, mean normal angle in degrees)
-    tol = {"bf16x3": (5e-6, 0.02), "fp16": (1e-4, 0.3), "bf16": (5e-4, 1.5)}
+    tol = {"bf16x3": (5e-6, 0.02), "bf16x3f": (5e-5, 0.15), "fp16": (1e-4, 0.3), "bf16": (5e-4, 1.5)}
     for prec, (tc, ta) in tol.items():
         m = _mesh(LevelStack(levels, prec).extract_volume(N, list(DELTAS), chunk=CHUNK)[0], N)
         da, ia = cKDTree(ref[0]).query(m[0])

@@ -70,6 +70,62 @@ def test_bf16x3_absolute_error_on_unit_outputs():
     assert np.abs(f1 - ref).max() > 1e-3
 
 
+# Three-term BF16 with the last hidden product in FP16 (split activations):
+# levels with three hidden layers up to width 256 take it; every other level
+# runs plain three-term BF16 (or FP16 above 256), bit for bit.
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_bf16x3f_forward(case):
+    hidden, omega, last = case
+    m = seeded_siren(hidden, omega, seed=1234 + len(hidden) + hidden[0], last_scale=last).cuda()
+    x64 = _points(10007, 7)
+    x = x64.float().cuda()
+    f = LevelStack([m], "bf16x3f").eval(x)
+    if len(hidden) != 3 or max(hidden) > 256:
+        assert torch.equal(f, LevelStack([m], "bf16x3").eval(x))
+        return
+    ref = osiren.forward(osiren.layers_from_model(m), x64.numpy())
+    err = np.abs(f.double().cpu().numpy() - ref).max()
+    scale = np.abs(ref).max()
+    e16 = np.abs(LevelStack([m], "fp16").eval(x).double().cpu().numpy() - ref).max()
+    print(f"bf16x3f {hidden}: max err {err:.3e} (fp16 {e16:.3e}), max |f| {scale:.3e}")
+    # the FP16 last product rounds only its weights (and the lo activations):
+    # below the error of the all-FP16 path, within 3e-4 of the output scale
+    assert err <= 3e-4 * scale
+    assert err < 0.8 * e16
+
+
+def test_bf16x3f_error_scales_with_the_output_weights():
+    # outputs up to ~3.5 (output weights x 30): the error of the FP16 last
+    # product grows with them, staying within 3e-4 of the output scale and
+    # below the all-FP16 path's (the three-term path: 5e-5 absolute here)
+    m = seeded_siren([256, 256, 256], 30.0, seed=99, last_scale=30.0).cuda()
+    x64 = _points(20000, 3)
+    ref = osiren.forward(osiren.layers_from_model(m), x64.numpy())
+    x = x64.float().cuda()
+    err = np.abs(LevelStack([m], "bf16x3f").eval(x).double().cpu().numpy() - ref).max()
+    e16 = np.abs(LevelStack([m], "fp16").eval(x).double().cpu().numpy() - ref).max()
+    print(f"bf16x3f max err {err:.3e} (fp16 {e16:.3e}), max |f| {np.abs(ref).max():.3e}")
+    assert err <= 3e-4 * np.abs(ref).max()
+    assert err < 0.8 * e16
+
+
+def test_bf16x3f_serves_values_only():
+    from mplicits import training
+    from mplicits.batch import BatchedExtraction
+    from mplicits.tracer import trace, renderer_camera
+    m = seeded_siren([256, 256, 256], 30.0, seed=5).cuda()
+    st = LevelStack([m], "bf16x3f")
+    x = _points(1000, 1).float().cuda()
+    with pytest.raises(NotImplementedError):
+        st.eval(x, grad=True)
+    with pytest.raises(NotImplementedError):
+        trace(st, [renderer_camera(64, 64)], 64, 64)
+    with pytest.raises(NotImplementedError):
+        BatchedExtraction([st], [32], [[]])
+    with pytest.raises(NotImplementedError):
+        training.level_forward(m, x, stack=st)
+
+
 def test_bf16x3_levels_mixed_with_fp16():
     levels = [seeded_siren([256, 256, 256], 30.0, seed=7).cuda(),
               seeded_siren([512, 512, 512], 90.0, seed=8, last_scale=0.1).cuda()]
