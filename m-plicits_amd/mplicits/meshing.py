@@ -12,7 +12,7 @@ from the reference, by design:
     synchronisations in total instead of two per chunk;
   * marching cubes runs on the GPU; a vertex inside a cube (tiling index 12)
     is created once per cube and shared by that cube's triangles;
-  * ``precision`` ("fp32", "bf16", "fp16", "bf16x3") selects the hidden-GEMM
+  * ``precision`` ("fp32", "bf16", "fp16", "bf16x3", "bf16x3f") selects the hidden-GEMM
     arithmetic.
 """
 import ctypes

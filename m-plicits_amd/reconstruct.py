@@ -3,7 +3,7 @@
 
 Same arguments, prints (model summaries, SAMPLING_TIME_S:<s>, Done) and
 outputs as reconstruct.py:36-133; the models are evaluated and meshed on the
-GPU by the HIP library.  Extra options: --precision (fp32 | bf16 | fp16 | bf16x3) and
+GPU by the HIP library.  Extra options: --precision (fp32 | bf16 | fp16 | bf16x3 | bf16x3f) and
 --grid (reference | lattice).
 """
 import argparse
@@ -57,7 +57,7 @@ def main(argv=None):
     parser.add_argument("--input", default=None, help="Training point cloud (.ply) for the adaptive deltas.")
     parser.add_argument("--deltas", type=float, nargs="+", default=None,
                         help="Explicit band widths (one per residual level).")
-    parser.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "bf16x3"],
+    parser.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "bf16x3", "bf16x3f"],
                         help="Arithmetic of the hidden-layer GEMMs.")
     parser.add_argument("--grid", default="reference", choices=["reference", "lattice"],
                         help="Sample coordinates: the reference's float32 grid, or the regular lattice.")
