@@ -644,7 +644,8 @@ def main():
     else:
         head = next((p for p in ("bf16x3f", "bf16x3") if p in results), next(iter(results)))
 
-    tr = bench_trace(levels, ["bf16", "fp16", "bf16x3", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
+    tr = bench_trace(levels, ["bf16", "fp16", "bf16x3", "bf16x3f", "fp32"],
+                     check=not args.no_cpu_baseline) if not args.no_extras else None
     if tr:
         print(f"# trace: {tr}", file=sys.stderr, flush=True)
     tn = bench_train(levels, ["bf16", "fp16", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None

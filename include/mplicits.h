@@ -51,10 +51,12 @@ extern "C" {
  * last one on the FP16 matrix cores with FP16 weights against the
  * activations as an FP16 pair (a*hi + a*lo: two MFMAs per step instead of
  * three).  Applies to levels with exactly three hidden layers up to width 256
- * (other levels as MP_BF16X3).  Values only: evaluation without gradients and
- * extraction; gradients, tracing, rendering, the batched extraction and the
- * training entry points return MP_EUNSUPPORTED.  The error of such a level
- * stays near 5e-5 where MP_BF16X3's is near 3e-6. */
+ * (other levels as MP_BF16X3).  Served by evaluation (gradients in forward
+ * mode), extraction and tracing (when every level up to width 256 has three
+ * hidden layers); rendering, the batched extraction and the training entry
+ * points return MP_EUNSUPPORTED.  The error of such a level stays near 5e-5
+ * on the fitted coarse level where MP_BF16X3's is near 3e-6, and grows with
+ * its output weights. */
 #define MP_BF16X3F 4
 
 /* sample coordinates of the extraction grid */

@@ -274,13 +274,9 @@ int launch_level_grad(const mp_stack* st, int level, const float* xyz, int64_t n
   // level (its operand pairs fill the registers the reverse mode would need)
   const int gemms = L.n_hidden - 1;
   const bool fp32 = shape == SH_16x16x4_F32;
-  if (shape == SH_32x32x16_BF16X3 ||
+  if (shape == SH_32x32x16_BF16X3 || shape == SH_32x32x16_BF16X3F ||
       (gemms >= 1 && (fp32 ? L.width >= 256 : (L.width == 512 || (L.width == 256 && gemms == 1)))))
     return launch_level_tangent(shape, L, xyz, n, f, g, accumulate, stream, n_dev);
-  if (shape == SH_32x32x16_BF16X3F) {
-    set_error("gradient: MP_BF16X3F (FP16 last product) serves values only; use MP_BF16X3 for gradients");
-    return MP_EUNSUPPORTED;
-  }
   switch (shape) {
     case SH_32x32x16_BF16: return launch_grad_width<S32x32x16_bf16>(L, xyz, n, f, g, accumulate, n_dev, stream);
     case SH_32x32x16_F16: return launch_grad_width<S32x32x16_f16>(L, xyz, n, f, g, accumulate, n_dev, stream);

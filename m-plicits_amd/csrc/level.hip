@@ -174,7 +174,13 @@ int launch_level_tangent(int shape, const LevelDev& L, const float* xyz, int64_t
       set_error("gradient launch: three-term BF16 supports padded widths up to 256, got %d", L.width);
       return MP_EUNSUPPORTED;
     case SH_32x32x16_BF16X3F:
-      set_error("gradient launch: MP_BF16X3F (FP16 last product) serves values only; use MP_BF16X3 for gradients");
+      MP_REQUIRE(L.n_hidden == 3, MP_EUNSUPPORTED, "gradient launch: the mixed three-term shape needs 3 hidden layers");
+      switch (L.width) {
+        case 64: return launch_tangent_typed<S32x32x16_bf16x3f, 64>(L, xyz, n, f, g, accumulate, n_dev, stream);
+        case 128: return launch_tangent_typed<S32x32x16_bf16x3f, 128>(L, xyz, n, f, g, accumulate, n_dev, stream);
+        case 256: return launch_tangent_typed<S32x32x16_bf16x3f, 256>(L, xyz, n, f, g, accumulate, n_dev, stream);
+      }
+      set_error("gradient launch: the mixed three-term shape supports padded widths up to 256, got %d", L.width);
       return MP_EUNSUPPORTED;
   }
   set_error("gradient launch: unsupported fragment shape %d", shape);

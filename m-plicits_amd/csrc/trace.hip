@@ -452,6 +452,16 @@ static int launch_trace(int blocks, size_t lds, hipStream_t s, const TraceArgs& 
   return MP_OK;
 }
 
+// The tracer instantiates one level shape per stack: an MP_BF16X3F stack is
+// traced when every level it runs in the mixed shape has that shape, i.e.
+// every level up to width 256 has three hidden layers.
+static bool mixed_levels_ok(const mp_stack* st) {
+  if (st->shape != SH_32x32x16_BF16X3F) return true;
+  for (int l = 0; l < st->n_levels; ++l)
+    if (st->lv[l].width <= 256 && st->lv[l].shape != SH_32x32x16_BF16X3F) return false;
+  return true;
+}
+
 }  // namespace mp
 
 using namespace mp;
@@ -468,8 +478,8 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
              st->n_levels - 1, prm->n_deltas);
   MP_REQUIRE(prm->max_steps >= 1 && prm->hit_eps > 0.f && prm->bound_radius > 0.f, MP_EINVAL,
              "mp_trace: bad parameters");
-  MP_REQUIRE(st->precision != MP_BF16X3F, MP_EUNSUPPORTED,
-             "mp_trace: MP_BF16X3F (FP16 last product) serves extraction and values only; use MP_BF16X3");
+  MP_REQUIRE(mixed_levels_ok(st), MP_EUNSUPPORTED,
+             "mp_trace: MP_BF16X3F traces stacks whose levels up to width 256 have 3 hidden layers; use MP_BF16X3");
   hipStream_t s = as_stream(stream);
   const int64_t total = (int64_t)n_cams * prm->width * prm->height;
   MP_REQUIRE(total < (1ll << 32), MP_EUNSUPPORTED, "mp_trace: too many rays");
@@ -521,7 +531,7 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
   // then run one block at a time, only where a ray needs them); three-term
   // BF16 tiles hold one
   const int want = st->trace_tiles ? st->trace_tiles : st->nc;
-  const int nc = want == 2 && st->shape != SH_32x32x16_BF16X3 ? 2 : 1;
+  const int nc = want == 2 && st->shape != SH_32x32x16_BF16X3 && st->shape != SH_32x32x16_BF16X3F ? 2 : 1;
   // persistent grid: one workgroup of four waves per CU (the LDS handoff
   // regions take 128 KB of the CU's LDS)
   const int blocks = cus;
@@ -538,6 +548,8 @@ int mp_trace(mp_stack* st, const float* cams, int n_cams, const mp_trace_params*
         return launch_trace<S16x16x4_f32, 1>(blocks, lds, s, ta);
       case SH_32x32x16_BF16X3:
         return launch_trace<S32x32x16_bf16x3, 1>(blocks, lds, s, ta);
+      case SH_32x32x16_BF16X3F:
+        return launch_trace<S32x32x16_bf16x3f, 1>(blocks, lds, s, ta);
       default:
         set_error("mp_trace: unsupported fragment shape %d", st->shape);
         return MP_EUNSUPPORTED;
@@ -615,8 +627,9 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
     const mp_trace_job& J = jobs[j];
     const mp_trace_params* prm = &J.params;
     MP_REQUIRE(J.stack != nullptr && J.cam != nullptr, MP_EINVAL, "mp_trace_batch_plan: job %d: NULL stack or camera", j);
-    MP_REQUIRE(J.stack->precision != MP_BF16X3F, MP_EUNSUPPORTED,
-               "mp_trace_batch_plan: job %d: MP_BF16X3F serves extraction and values only; use MP_BF16X3", j);
+    MP_REQUIRE(mixed_levels_ok(J.stack), MP_EUNSUPPORTED,
+               "mp_trace_batch_plan: job %d: MP_BF16X3F traces stacks whose levels up to width 256 have 3 hidden "
+               "layers; use MP_BF16X3", j);
     MP_REQUIRE(J.stack->shape == shape, MP_EINVAL, "mp_trace_batch_plan: job %d: every stack must have the same precision", j);
     MP_REQUIRE(J.depth != nullptr && J.steps != nullptr && J.rgba != nullptr, MP_EINVAL,
                "mp_trace_batch_plan: job %d: depth, steps and rgba are required", j);
@@ -675,7 +688,8 @@ int mp_trace_batch_plan(const mp_trace_job* jobs, int n_jobs, void* stream, mp_t
       jw = J.stack->lv[l].width > jw ? J.stack->lv[l].width : jw;
     }
     const int want = J.stack->trace_tiles ? J.stack->trace_tiles : J.stack->nc;
-    a.cols = (want == 2 && jw <= 256 && J.stack->shape != SH_32x32x16_BF16X3) ? 2 : 1;
+    a.cols = (want == 2 && jw <= 256 && J.stack->shape != SH_32x32x16_BF16X3 &&
+              J.stack->shape != SH_32x32x16_BF16X3F) ? 2 : 1;
     for (int i = 0; i < J.params.n_deltas; ++i) a.deltas[i] = J.params.deltas[i];
     unsigned long long* c = p->ctr + (size_t)j * TB_CTR;
     a.next_ray = &c[0];
@@ -725,6 +739,7 @@ int mp_trace_batch_run(mp_trace_batch* p, void* stream) {
     case SH_32x32x16_F16: rc = launch_trace_jobs<S32x32x16_f16>(p->blocks, lds, s, jobs, p->n_jobs); break;
     case SH_16x16x4_F32: rc = launch_trace_jobs<S16x16x4_f32>(p->blocks, lds, s, jobs, p->n_jobs); break;
     case SH_32x32x16_BF16X3: rc = launch_trace_jobs<S32x32x16_bf16x3>(p->blocks, lds, s, jobs, p->n_jobs); break;
+    case SH_32x32x16_BF16X3F: rc = launch_trace_jobs<S32x32x16_bf16x3f>(p->blocks, lds, s, jobs, p->n_jobs); break;
     default:
       set_error("mp_trace_batch_run: unsupported fragment shape %d", p->shape);
       return MP_EUNSUPPORTED;

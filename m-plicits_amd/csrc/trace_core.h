@@ -24,6 +24,10 @@ template <>
 struct WideShape<S32x32x16_bf16x3> {
   typedef S32x32x16_f16 type;
 };
+template <>
+struct WideShape<S32x32x16_bf16x3f> {
+  typedef S32x32x16_f16 type;
+};
 
 // f_level at the NC points of this lane; hand: the wave's handoff region.
 // A 512-wide level runs one column block at a time (its operands fill the

@@ -21,7 +21,7 @@ MP_FP32 = 0
 MP_BF16 = 1
 MP_FP16 = 2
 MP_BF16X3 = 3  # three-term BF16 (hi.hi + hi.lo + lo.hi); no batched extraction or training
-MP_BF16X3F = 4  # three-term BF16 with the last hidden product in FP16 (split activations); values only
+MP_BF16X3F = 4  # three-term BF16 with the last hidden product in FP16 (split activations); no render/batch/train
 PRECISIONS = {"fp32": MP_FP32, "bf16": MP_BF16, "fp16": MP_FP16, "bf16x3": MP_BF16X3, "bf16x3f": MP_BF16X3F}
 
 MP_GRID_REFERENCE = 0

@@ -56,8 +56,8 @@ class SIREN(nn.Module):
     by the kernels as ``sin(w * (W x + b))``; checkpoints saved after
     ``update_omegas(1)`` carry them baked into the weights.
 
-    Extra attribute ``precision`` ("fp32", "bf16", "fp16", "bf16x3" or "bf16x3f", the
-    last for values only) selects the arithmetic of the hidden-layer GEMMs on the GPU.
+    Extra attribute ``precision`` ("fp32", "bf16", "fp16", "bf16x3" or "bf16x3f")
+    selects the arithmetic of the hidden-layer GEMMs on the GPU.
     """
 
     def __init__(self, n_in_features, n_out_features, hidden_layer_config=[], w0=30, ww=None,

@@ -47,7 +47,8 @@ class LevelStack:
         with the last of a three-hidden-layer level's two products on the FP16
         matrix cores (FP16 weights against the activations as an FP16 pair:
         two MFMAs per step instead of three, level error near 5e-5); it serves
-        values only (evaluation without gradients and extraction).
+        evaluation (gradients in forward mode), extraction and tracing, not
+        rendering, batched extraction or training.
     """
 
     def __init__(self, models, precision="fp32"):
