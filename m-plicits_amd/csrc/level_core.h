@@ -29,8 +29,6 @@
 //    overlap as far as one wave's in-order issue allows.
 #pragma once
 
-#include <type_traits>
-
 #include "eval_dispatch.h"
 
 namespace mp {
@@ -415,18 +413,18 @@ struct LevelCore {
     A.begin();
 
     // hidden layers that feed another hidden layer: B registers -> LDS -> B registers
-    auto hidden = [&](int j, auto f16_out) {
-      constexpr bool H16 = decltype(f16_out)::value;
+    if constexpr (LF) {
+      // the one hidden-to-hidden product (the stack gives LAST_F16 only to
+      // levels with n_gemm == 2) hands over FP16 pairs
       Frag st[NC][BP];
-      gemm(A, j, L.biasc + (size_t)j * W, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
+      gemm(A, 0, L.biasc, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
         const int t = rb * CREGS + q;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          if constexpr (H16) set2h(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
-          else set2(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
+          set2h(st[c][0], st[c][1], t % EPL, act_rev(acc[c][q], lane));
           if (t % EPL == EPL - 1) {
             hand[(c * KBS + t / EPL) * 64 + lane] = st[c][0];
-            if constexpr (SPLIT) hand[((NC + c) * KBS + t / EPL) * 64 + lane] = st[c][1];
+            hand[((NC + c) * KBS + t / EPL) * 64 + lane] = st[c][1];
           }
         }
       });
@@ -435,13 +433,30 @@ struct LevelCore {
 #pragma unroll
         for (int kb = 0; kb < KBS; ++kb) {
           b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
-          if constexpr (SPLIT) b[c][LO + kb] = hand[((NC + c) * KBS + kb) * 64 + lane];
+          b[c][LO + kb] = hand[((NC + c) * KBS + kb) * 64 + lane];
         }
-    };
-    if constexpr (LF) {
-      hidden(0, std::true_type());  // the stack gives LAST_F16 only to levels with n_gemm == 2
     } else {
-      for (int j = 0; j + 1 < n_gemm; ++j) hidden(j, std::false_type());
+      for (int j = 0; j + 1 < n_gemm; ++j) {
+        Frag st[NC][BP];
+        gemm(A, j, L.biasc + (size_t)j * W, b, g, [&](int rb, int q, const Acc (&acc)[NC]) {
+          const int t = rb * CREGS + q;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            set2(st[c][0], st[c][BP - 1], t % EPL, act_rev(acc[c][q], lane));
+            if (t % EPL == EPL - 1) {
+              hand[(c * KBS + t / EPL) * 64 + lane] = st[c][0];
+              if constexpr (SPLIT) hand[((NC + c) * KBS + t / EPL) * 64 + lane] = st[c][1];
+            }
+          }
+        });
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int kb = 0; kb < KBS; ++kb) {
+            b[c][kb] = hand[(c * KBS + kb) * 64 + lane];
+            if constexpr (SPLIT) b[c][LO + kb] = hand[((NC + c) * KBS + kb) * 64 + lane];
+          }
+      }
     }
 
     // last hidden layer fused with the output layer (output weights of row
