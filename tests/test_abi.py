@@ -28,6 +28,19 @@ def test_python_binding_covers_every_symbol():
     assert set(_declared_symbols()) == set(_lib.SIGNATURES)
 
 
+def test_header_constants_match_the_binding():
+    # every precision, tile family and error code the header defines has the
+    # same value in the Python binding (the precision names map onto them)
+    from mplicits import _lib
+    text = open(osp.join(ROOT, "include", "mplicits.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define (MP_[A-Z0-9_]+) \(?(-?\d+)\)?", text)}
+    for name in ("MP_FP32", "MP_BF16", "MP_FP16", "MP_BF16X3", "MP_BF16X3F", "MP_OK", "MP_EINVAL",
+                 "MP_EUNSUPPORTED", "MP_TILES_EXTRACT", "MP_TILES_BATCH", "MP_TILES_TRACE"):
+        assert getattr(_lib, name) == defs[name], name
+    assert _lib.PRECISIONS == {"fp32": defs["MP_FP32"], "bf16": defs["MP_BF16"], "fp16": defs["MP_FP16"],
+                               "bf16x3": defs["MP_BF16X3"], "bf16x3f": defs["MP_BF16X3F"]}
+
+
 def test_abi_version_and_error_path(lib):
     from mplicits import _lib
     assert lib.mp_abi_version() == 1
