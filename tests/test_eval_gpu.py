@@ -73,7 +73,7 @@ def test_bf16x3_absolute_error_on_unit_outputs():
 # Three-term BF16 with the last hidden product in FP16 (split activations):
 # levels with three hidden layers up to width 256 take it; every other level
 # runs plain three-term BF16 (or FP16 above 256), bit for bit.
-@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])))
+@pytest.mark.parametrize("case", CASES + [([64, 64, 64], 30.0, 1.0)], ids=lambda c: "x".join(map(str, c[0])))
 def test_bf16x3f_forward(case):
     hidden, omega, last = case
     m = seeded_siren(hidden, omega, seed=1234 + len(hidden) + hidden[0], last_scale=last).cuda()
