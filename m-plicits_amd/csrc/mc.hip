@@ -37,6 +37,9 @@ namespace mp {
 
 static constexpr double MC_EPS = 2.220446049250313e-16;
 static constexpr int MC_BLOCK = 256;
+// vertex counts below this keep a point's edge flags in the top bits of its
+// vertex-base word
+static constexpr uint32_t MC_PACK_LIMIT = 1u << 28;
 
 // corner p -> (dx, dy, dz)
 __device__ static const int8_t MC_CX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
@@ -595,6 +598,7 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, const uint32_
                                                           const uint32_t* __restrict__ off, uint32_t* vbase,
                                                           float* verts, float* normals, float* values) {
   const int64_t na = (int64_t)*na_dev;
+  const bool pack = na_dev[2] < MC_PACK_LIMIT;  // vertex count
   for (int64_t k = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x; k < na; k += (int64_t)gridDim.x * MC_BLOCK) {
     const uint32_t i = list[k];
     int x, y, z;
@@ -606,7 +610,7 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, const uint32_
     // only points that own vertices are ever looked up (an edge a triangle
     // uses has its owner's flag set; a centre vertex belongs to this point)
     const uint32_t base = off[2 * k];
-    vbase[i] = base;
+    vbase[i] = pack ? base | ((uint32_t)p.flags << 28) : base;
     const double v0 = mc_value(V, x, y, z);
     uint32_t id = base;
     for (int ax = 0; ax < 3; ++ax) {
@@ -678,12 +682,17 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_vertices(McVol V, const uint32_
   }
 }
 
-__device__ inline uint32_t mc_vertex_id(const McVol& V, const uint32_t* vbase, int x, int y, int z, int e) {
-  if (e == 12) return vbase[((int64_t)z * V.ny + y) * V.nx + x] + __popc(mc_edge_flags(V, x, y, z));
-  const int ox = x + MC_OX[e], oy = y + MC_OY[e], oz = z + MC_OZ[e];
-  const int ax = MC_AX[e];
-  const int f = mc_edge_flags(V, ox, oy, oz);
-  return vbase[((int64_t)oz * V.ny + oy) * V.nx + ox] + __popc(f & ((1 << ax) - 1));
+// Vertex id of entry e (edge 0..11, or 12: the cell's centre vertex, after
+// the point's own edge vertices).  vbase[owner] holds the owner's first vertex
+// id and, when the vertex count leaves room (pack), its edge flags in bits
+// 28..30: then no volume value of the owner is read.
+__device__ inline uint32_t mc_vertex_id(const McVol& V, const uint32_t* vbase, bool pack, int x, int y, int z, int e) {
+  const int ox = e == 12 ? x : x + MC_OX[e], oy = e == 12 ? y : y + MC_OY[e], oz = e == 12 ? z : z + MC_OZ[e];
+  const int ax = e == 12 ? 3 : MC_AX[e];
+  const uint32_t w = vbase[((int64_t)oz * V.ny + oy) * V.nx + ox];
+  const uint32_t base = pack ? (w & (MC_PACK_LIMIT - 1u)) : w;
+  const int f = pack ? (int)(w >> 28) : mc_edge_flags(V, ox, oy, oz);
+  return base + __popc(f & ((1 << ax) - 1));
 }
 
 // Pass 7: faces of every active point, vertex ids looked up in vbase.
@@ -692,6 +701,7 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_faces(McVol V, const uint32_t* 
                                                        const uint32_t* __restrict__ off, const uint32_t* vbase,
                                                        int32_t* faces) {
   const int64_t na = (int64_t)*na_dev;
+  const bool pack = na_dev[2] < MC_PACK_LIMIT;  // vertex count
   for (int64_t k = (int64_t)blockIdx.x * MC_BLOCK + threadIdx.x; k < na; k += (int64_t)gridDim.x * MC_BLOCK) {
     int x, y, z;
     mc_xyz_of(V, list[k], x, y, z);
@@ -710,7 +720,7 @@ __global__ __launch_bounds__(MC_BLOCK) void k_mc_faces(McVol V, const uint32_t* 
     for (int kk = 0; kk < nt; ++kk) {
       // 'descent': faces in reversed table order
       for (int j = 0; j < 3; ++j)
-        faces[3 * (fb + kk) + j] = (int32_t)mc_vertex_id(V, vbase, x, y, z, tri[3 * kk + 2 - j]);
+        faces[3 * (fb + kk) + j] = (int32_t)mc_vertex_id(V, vbase, pack, x, y, z, tri[3 * kk + 2 - j]);
     }
   }
 }
