@@ -95,6 +95,27 @@ def test_trace_batch_matches_single_model_trace(precision):
     b.close()
 
 
+def test_trace_batch_bf16x3f_matches_single_model_trace():
+    # the mixed three-term path in the batched tracer: every level up to
+    # width 256 with three hidden layers (the 512-wide residual runs FP16)
+    from mplicits.synthetic import orbit_cameras
+    from mplicits.tracer import BatchedTrace, trace
+    models = [_model(64, 1, 400), _model(128, 1, 401), _model(256, 1, 402),
+              _model(256, 1, 403) + [seeded_siren([512] * 2, 60.0, seed=404, last_scale=0.1).cuda()]]
+    sizes, deltas = [(40, 24), (33, 20), (36, 22), (48, 27)], [[], [], [], [0.05]]
+    stacks = [LevelStack(m, "bf16x3f") for m in models]
+    cams = orbit_cameras(len(models), seed=6, aspect=1.5)
+    refs = [trace(st, [cam], W, H, deltas=d, max_steps=64) for st, cam, (W, H), d in zip(stacks, cams, sizes, deltas)]
+    b = BatchedTrace([(st, cam, W, H, d) for st, cam, (W, H), d in zip(stacks, cams, sizes, deltas)], max_steps=64)
+    for rep in range(2):
+        b.run(graph=rep > 0)
+        torch.cuda.synchronize()
+        for j, ref in enumerate(refs):
+            for k in ("depth", "steps", "rgba", "normals"):
+                assert torch.equal(getattr(b, k)[j], ref[k][0]), (rep, j, k)
+    b.close()
+
+
 def test_autotune_caches_and_applies(tmp_path):
     from mplicits import autotune
     cache = str(tmp_path / "tiles.txt")
