@@ -507,6 +507,14 @@ def train_batch(levels, n, seed):
     return x, shape.sdf(x).float(), shape.normals(x).float()
 
 
+def train_precisions(key):
+    """(training GEMM precision, frozen-level precision) of a training path:
+    "fp16" trains and evaluates the frozen levels in FP16, "fp16/bf16x3"
+    trains in FP16 over frozen levels evaluated in three-term BF16."""
+    train, _, frozen = key.partition("/")
+    return train, frozen or train
+
+
 def train_grad_errors(levels, model, x, f_t, n_t, precision, n=1 << 14):
     """CPU leg (checker): the parameter gradients of one step on the first n
     points of the batch against the oracle's FP64 autograd (double backward
@@ -527,7 +535,8 @@ def train_grad_errors(levels, model, x, f_t, n_t, precision, n=1 << 14):
 
     for p in model.parameters():
         p.grad = None
-    training.train_step(model, xs, loss_fn, frozen=LevelStack([coarse, med], precision), precision=precision)
+    tp, fp = train_precisions(precision)
+    training.train_step(model, xs, loss_fn, frozen=LevelStack([coarse, med], fp), precision=tp)
     dev = lambda m: [(torch.from_numpy(w).cuda(), torch.from_numpy(b).cuda())  # noqa: E731
                      for w, b in osiren.layers_from_model(m)]
     fl = [(w.requires_grad_(True), b.requires_grad_(True)) for w, b in dev(model)]
@@ -552,25 +561,27 @@ def bench_train(levels, precisions, steps=4, check=False):
     """One Adam step of the fine 512x3 residual over the frozen coarse + medium
     levels on 2^18 points (half uniform in [-1,1]^3, half in the fine band),
     config loss (L1 SDF + eikonal + normal cosine): training points per second;
-    with check (CPU leg) the parameter gradients' error against FP64."""
+    with check (CPU leg) the parameter gradients' error against FP64.
+    precisions: training paths as train_precisions reads them."""
     coarse, med, fine = levels
     n = 2 ** 18
     x, f_t, n_t = train_batch(levels, n, MASTER_SEED)
     out = {}
     for prec in precisions:
-        frozen = LevelStack([coarse, med], prec)
+        tp, fp = train_precisions(prec)
+        frozen = LevelStack([coarse, med], fp)
         model = type(fine)(3, 1, fine.hidden_widths(), w0=1, ww=1, delay_init=True).cuda()
         model.load_state_dict(fine.state_dict())
         opt = torch.optim.Adam(model.parameters(), lr=1e-4)
         loss_fn = lambda F, G: training.config_loss(F, G, f_t, n_t)  # noqa: E731
         for _ in range(2):
             opt.zero_grad(set_to_none=True)
-            training.train_step(model, x, loss_fn, frozen=frozen, optimizer=opt, precision=prec)
+            training.train_step(model, x, loss_fn, frozen=frozen, optimizer=opt, precision=tp)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             opt.zero_grad(set_to_none=True)
-            loss, _, _ = training.train_step(model, x, loss_fn, frozen=frozen, optimizer=opt, precision=prec)
+            loss, _, _ = training.train_step(model, x, loss_fn, frozen=frozen, optimizer=opt, precision=tp)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         out[prec] = {"train_pts_per_s": round(n / dt, 1), "ms_per_step": round(dt * 1e3, 2),
@@ -648,7 +659,7 @@ def main():
                      check=not args.no_cpu_baseline) if not args.no_extras else None
     if tr:
         print(f"# trace: {tr}", file=sys.stderr, flush=True)
-    tn = bench_train(levels, ["bf16", "fp16", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
+    tn = bench_train(levels, ["bf16", "fp16", "fp16/bf16x3", "fp32"], check=not args.no_cpu_baseline) if not args.no_extras else None
     if tn:
         print(f"# train: {tn}", file=sys.stderr, flush=True)
     td = bench_train_data(levels) if not args.no_extras else None

@@ -320,7 +320,7 @@ void mp_trace_batch_destroy(mp_trace_batch* plan);
  *     biases; dL_df [n] or dL_dgrad [n][3] may be NULL (zero).  The layer
  *     activations are recomputed unless kept by mp_train_fwd.  In FP16 the
  *     seeds are scaled on the device by the power of two that brings
- *     max |seed| into [16, 32) and the gradients back by its inverse (exact),
+ *     max |seed| into [2^10, 2^11) and the gradients back by its inverse (exact),
  *     so that the FP16 adjoints neither flush to zero nor lose precision.
  * Both allocate the training workspace on first use (about 16 n W (2 H - 1)
  * bytes for H sine layers of width W in 16 bit, up to twice that in FP32) and

@@ -202,9 +202,14 @@ __global__ void k_tw_seeds(const float* __restrict__ s, const float* __restrict_
 // the seeds of a mean loss over n points are O(1/n), and in FP16 (smallest
 // normal 6.1e-5) such adjoints lose their precision or flush to zero.  The
 // seeds are therefore scaled by a power of two S that brings max |seed| into
-// [2^4, 2^5) (room for the adjoints' growth through the layers below FP16's
-// 65504), and the parameter gradients by 1/S at the end; both scalings are
-// exact.  Block maxima of |sg| into red[0..255]:
+// [2^10, 2^11), and the parameter gradients by 1/S at the end; both scalings
+// are exact.  The pre-activation adjoints are the seeds times the output
+// weights and cosines, so they sit well below the seeds (max |adjoint| /
+// max |seed| ~ 0.006 on a 512x3 level with output weights x 0.1, ~0.06 with
+// unscaled ones) and grow slowly through the layers: 2^10 keeps the small
+// ones out of the subnormal range (at 2^4 the worst parameter tensor's error
+// was 1.6x larger) with room up to FP16's 65504.  Block maxima of |sg| into
+// red[0..255]:
 __global__ __launch_bounds__(256) void k_tw_absmax(const float* __restrict__ src, int64_t n, float* __restrict__ out) {
   __shared__ float red[256];
   float m = 0.f;
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(256) void k_tw_seed_scale(const float* __restrict__
     const float m = red[0];
     int e = 0;
     if (m > 0.f) (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
-    int k = m > 0.f ? 5 - e : 0;
+    int k = m > 0.f ? 11 - e : 0;  // m S in [2^10, 2^11)
     k = k < -100 ? -100 : (k > 100 ? 100 : k);
     scl[0] = ldexpf(1.f, k);
     scl[1] = ldexpf(1.f, -k);
