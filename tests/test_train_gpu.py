@@ -193,6 +193,39 @@ def test_train_gradients_with_mean_loss_seeds(precision, tol):
         assert torch.equal(dp32, dp * 32)
 
 
+@pytest.mark.parametrize("last_scale", [1.0, 5.0])
+def test_fp16_seed_scaling_headroom(last_scale):
+    """The FP16 backward brings max |seed| into [2^10, 2^11); the adjoints are
+    the seeds times the output weights, so large output weights (x1, x5 the
+    initialisation's) and a seed far above the rest must still leave every
+    FP16 adjoint finite: gradients finite and within 1e-2 of FP64 autograd
+    (max error per tensor relative to its max) on a 512x3 level."""
+    m = seeded_siren([512] * 3, 90.0, seed=72, last_scale=last_scale).cuda()
+    st = LevelStack([m], "fp16")
+    st.refresh()
+    gen = torch.Generator().manual_seed(7)
+    n = 4099
+    x = (torch.rand(n, 3, generator=gen) * 2 - 1).cuda()
+    s = torch.randn(n, generator=gen) * 2.0 ** -12
+    g = torch.randn(n, 3, generator=gen) * 2.0 ** -12
+    s[17] = 2.0 ** -4  # one seed 2^8 x the others' scale
+    s, g = s.cuda(), g.cuda()
+    _, _, dp = _abi_train(st, x, s, g, keep=1)
+    assert torch.isfinite(dp).all()
+    lay = [(torch.from_numpy(w).requires_grad_(True), torch.from_numpy(b).requires_grad_(True))
+           for w, b in osiren.layers_from_model(m)]
+    fr, gr = olosses.level_value_grad(lay, x.double().cpu(), True)
+    obj = (s.double().cpu()[:, None] * fr).sum() + (g.double().cpu() * gr).sum()
+    ref = torch.autograd.grad(obj, [w for w, _ in lay] + [b for _, b in lay])
+    off = 0
+    for r in ref:
+        got = dp[off:off + r.numel()].double().cpu().view_as(r)
+        off += r.numel()
+        err = ((got - r).abs().max() / r.abs().max()).item()
+        print(f"fp16 x{last_scale} {tuple(r.shape)}: max rel err {err:.2e}")
+        assert err <= 1e-2, (tuple(r.shape), err)
+
+
 def test_train_entry_points_check_their_arguments():
     from mplicits import _lib
     lib = _lib.load()
