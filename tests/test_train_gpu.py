@@ -159,7 +159,7 @@ def test_train_entry_points_against_fp64_autograd(hidden, precision, tol):
     assert torch.equal(dp, dp2)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp16", 2e-3), ("bf16", 2e-2), ("fp32", 1e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp16", 1.5e-3), ("bf16", 2e-2), ("fp32", 1e-5)])
 def test_train_gradients_with_mean_loss_seeds(precision, tol):
     """Seeds of a mean loss over many points are O(1/n) (here 2^-16 .. 2^-14):
     FP16 adjoints would flush to zero or keep few bits without the backward's
