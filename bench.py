@@ -591,6 +591,43 @@ def bench_train(levels, precisions, steps=4, check=False):
     return out
 
 
+def fastest_within_bounds(results, head, errs, tr, tn):
+    """The fastest path per workload that meets the accuracy pairing, with its
+    speed-up over FP32: extraction (the headline path: max |SDF - FP64| <=
+    SDF_BOUND over the grid), tracing of the coarse model and of the
+    three-level stack (evaluation error <= SDF_BOUND at every hit and the same
+    arithmetic within SDF_BOUND over the extraction grid), the training step
+    (every parameter tensor's gradient within 1e-3 of FP64).  results: {path:
+    {"ms"}}, errs: full-grid errors (None without the checker leg), tr / tn:
+    the tracing and training figures (None when skipped)."""
+    b = results[head]
+    fp32 = results.get("fp32")
+    pairings = {"extraction": {"path": head, "ms": round(b["ms"], 3),
+                               "max_sdf_err": (errs or {}).get(head, {}).get("max"),
+                               "vs_fp32": round(fp32["ms"] / b["ms"], 2) if fp32 else None}}
+    if tr and errs:
+        # a tracing path counts when its evaluation error at every hit and the same arithmetic's
+        # error over the whole extraction grid are both within the bound
+        for name, pre in (("trace_16x1080p_coarse", ""), ("trace_16x1080p_stack3", "stack3_")):
+            ok = {k[len(pre):]: v for k, v in tr.items() if (k.startswith(pre) if pre else "_" not in k)
+                  and v.get("vs_fp64", {}).get("eval_err_max", 1.0) <= SDF_BOUND
+                  and errs.get(k[len(pre):], {}).get("max", 1.0) <= SDF_BOUND}
+            ref = tr.get(pre + "fp32")
+            if ok and ref:
+                k = min(ok, key=lambda q: ok[q]["ms"])
+                pairings[name] = {"path": k, "rays_per_s": ok[k]["rays_per_s"],
+                                  "eval_err_max": ok[k]["vs_fp64"]["eval_err_max"],
+                                  "vs_fp32": round(ref["ms"] / ok[k]["ms"], 2)}
+    if tn and "fp32" in tn:
+        ok = {k: v for k, v in tn.items() if v.get("vs_fp64", {}).get("param_grad_rel_err_max_tensor", 1.0) <= 1e-3}
+        if ok:
+            k = min(ok, key=lambda q: ok[q]["ms_per_step"])
+            pairings["train_step_2e18"] = {"path": k, "ms_per_step": ok[k]["ms_per_step"],
+                                           "grad_rel_err_max_tensor": ok[k]["vs_fp64"]["param_grad_rel_err_max_tensor"],
+                                           "vs_fp32": round(tn["fp32"]["ms_per_step"] / ok[k]["ms_per_step"], 2)}
+    return pairings
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -710,29 +747,7 @@ def main():
 
     b = results[head]
     fp32 = results.get("fp32")
-    pairings = {"extraction": {"path": head, "ms": round(b["ms"], 3),
-                               "max_sdf_err": (errs or {}).get(head, {}).get("max"),
-                               "vs_fp32": round(fp32["ms"] / b["ms"], 2) if fp32 else None}}
-    if tr and errs:
-        # a tracing path counts when its evaluation error at every hit and the same arithmetic's
-        # error over the whole extraction grid are both within the bound
-        for name, pre in (("trace_16x1080p_coarse", ""), ("trace_16x1080p_stack3", "stack3_")):
-            ok = {k[len(pre):]: v for k, v in tr.items() if (k.startswith(pre) if pre else "_" not in k)
-                  and v.get("vs_fp64", {}).get("eval_err_max", 1.0) <= SDF_BOUND
-                  and errs.get(k[len(pre):], {}).get("max", 1.0) <= SDF_BOUND}
-            ref = tr.get(pre + "fp32")
-            if ok and ref:
-                k = min(ok, key=lambda q: ok[q]["ms"])
-                pairings[name] = {"path": k, "rays_per_s": ok[k]["rays_per_s"],
-                                  "eval_err_max": ok[k]["vs_fp64"]["eval_err_max"],
-                                  "vs_fp32": round(ref["ms"] / ok[k]["ms"], 2)}
-    if tn and "fp32" in tn:
-        ok = {k: v for k, v in tn.items() if v.get("vs_fp64", {}).get("param_grad_rel_err_max_tensor", 1.0) <= 1e-3}
-        if ok:
-            k = min(ok, key=lambda q: ok[q]["ms_per_step"])
-            pairings["train_step_2e18"] = {"path": k, "ms_per_step": ok[k]["ms_per_step"],
-                                           "grad_rel_err_max_tensor": ok[k]["vs_fp64"]["param_grad_rel_err_max_tensor"],
-                                           "vs_fp32": round(tn["fp32"]["ms_per_step"] / ok[k]["ms_per_step"], 2)}
+    pairings = fastest_within_bounds(results, head, errs, tr, tn)
     out = {
         "metric": "512^3 band-culled extraction ms, rays/s, train pts/s: BF16 vs FP32 at SDF err",
         "value": round(b["ms"], 3),
