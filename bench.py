@@ -732,6 +732,12 @@ def main():
         issued = coarse_flop + sum(e * p for e, p in zip(extra, prods)) * N ** 3
         roofline["issued_mfma_tflops"] = round(issued / (k_ms * 1e-3) / 1e12, 1)
         roofline["issued_frac"] = round(issued / (k_ms * 1e-3) / 1e12 / peak, 4)
+    if head != "bf16" and "bf16" in results:
+        # the same launch with one-term BF16 products (model FLOP = issued FLOP)
+        b_ms = dominant_kernel_ms(levels[0], "bf16")
+        roofline["bf16_same_launch"] = {"kernel_ms": round(b_ms, 3),
+                                        "achieved": round(coarse_flop / (b_ms * 1e-3) / 1e12, 1),
+                                        "frac": round(coarse_flop / (b_ms * 1e-3) / 1e12 / PEAK_TFLOPS["bf16"], 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
