@@ -17,7 +17,6 @@ import torch
 sys.path.insert(0, osp.dirname(osp.abspath(__file__)))
 
 from mplicits.meshing import create_mesh_multistage  # noqa: E402
-from mplicits.stack import LevelStack  # noqa: E402
 from mplicits.util import from_pth  # noqa: E402
 
 
@@ -25,19 +24,12 @@ def adaptive_deltas(models, input_ply, device, eps=0.3, batch=2 ** 18):
     """delta_i = (1 + eps) * max_j |F_i(x_j)| over the training points, F_i the
     partial sum up to level i (reconstruct.py:18-33, Eq. 5 of the paper).
     ``input_ply``: path of a PLY point cloud (x, y, z vertex properties)."""
+    from mplicits.deltas import partial_sum_maxima
     from mplicits.ply import read_ply
     v = read_ply(input_ply)["vertex"]
     pts = torch.from_numpy(np.stack([v["x"], v["y"], v["z"]], 1).astype(np.float32)).to(device)
-    stack = LevelStack(models, precision=getattr(models[0], "precision", "fp32"))
-    deltas = []
-    with torch.no_grad():
-        for level in range(1, len(models)):
-            worst = 0.0
-            for i in range(0, pts.shape[0], batch):
-                s = stack.eval(pts[i:i + batch], 0, level)
-                worst = max(worst, float(s.abs().max()))
-            deltas.append((1.0 + eps) * worst)
-    return deltas
+    maxima = partial_sum_maxima(models, pts, getattr(models[0], "precision", "fp32"), batch, count=len(models) - 1)
+    return [(1.0 + eps) * m for m in maxima]
 
 
 def main(argv=None):

@@ -1,6 +1,7 @@
 """The Python drop-ins called by their reference names: create_mesh_multistage
 (i3d/meshing.py:135-264), the reconstruct.py CLI with adaptive_deltas
-(reconstruct.py:18-133) and true_sdf (i3d/loss_functions.py:123-163), against
+(reconstruct.py:18-133), the Eq. 5 table of appendix_experiments/
+extract_deltas.py and true_sdf (i3d/loss_functions.py:123-163), against
 the oracle."""
 import os.path as osp
 import importlib.util
@@ -137,6 +138,44 @@ def test_adaptive_deltas_against_the_oracle(tmp_path, capsys):
               paths["medium"], "-r", "24", "--device", "cuda", "--multistage", "--input", CLOUD])
     text = capsys.readouterr().out
     assert "adaptive band widths (Eq. 5): " + ", ".join(f"{d:.3e}" for d in got) in text
+
+
+def test_extract_deltas_cli_against_the_oracle(tmp_path, capsys):
+    """The appendix table of Eq. 5 band widths (appendix_experiments/
+    extract_deltas.py): checkpoints found under the i3d layout, a shape
+    without them skipped, max |f_1| and max |f_1 + r_1| over the input points
+    against the oracle's FP64 partial sums, the printed line and the table."""
+    import shutil
+    spec = importlib.util.spec_from_file_location("extract_deltas_cli",
+                                                  osp.join(ROOT, "m-plicits_amd", "extract_deltas.py"))
+    cli = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(cli)
+    coarse, med, _ = _levels()
+    for stage, m in (("coarse", coarse), ("medium", med)):
+        d = tmp_path / "results" / "bumpy" / stage
+        d.mkdir(parents=True)
+        torch.save(m.state_dict(), str(d / "best.pth"))
+    (tmp_path / "results" / "half" / "coarse").mkdir(parents=True)  # no medium checkpoint
+    torch.save(coarse.state_dict(), str(tmp_path / "results" / "half" / "coarse" / "best.pth"))
+    inp = tmp_path / "data" / "normalized_sphere" / "input"
+    inp.mkdir(parents=True)
+    shutil.copy(CLOUD, str(inp / "bumpy.ply"))
+    table = tmp_path / "deltas_table.md"
+    rows = cli.main(["--root", str(tmp_path), "--shapes", "bumpy", "half", "--out", str(table)])
+    out = capsys.readouterr().out
+    v = read_ply(CLOUD)["vertex"]
+    pts = np.stack([v["x"], v["y"], v["z"]], 1).astype(np.float64)
+    f1 = osiren.forward(osiren.layers_from_model(coarse), pts)
+    r1 = osiren.forward(osiren.layers_from_model(med), pts)
+    m1, m2 = np.abs(f1).max(), np.abs(f1 + r1).max()
+    assert len(rows) == 1 and rows[0][0] == "bumpy" and rows[0][1] == len(pts)
+    np.testing.assert_allclose(rows[0][2:], [m1, 1.3 * m1, m2, 1.3 * m2], rtol=1e-5)
+    assert "[skip] half: missing checkpoints" in out
+    r = rows[0]
+    assert f"bumpy: n={len(pts)}  max|f1|={r[2]:.3e} d1={r[3]:.3e}  max|f1+r1|={r[4]:.3e} d2={r[5]:.3e}" in out
+    text = table.read_text()
+    assert text.startswith("# Adaptive band widths (Eq. 5, eps=0.3) from paper checkpoints")
+    assert f"| bumpy | {len(pts)} | {r[2]:.3e} | {r[3]:.3e} | {r[4]:.3e} | {r[5]:.3e} |" in text
 
 
 def test_true_sdf_against_the_oracle():
