@@ -110,20 +110,18 @@ __device__ inline uint32_t pack2(float lo, float hi) {
 // row t of units 0..3 (x = units 0, 1; y = units 2, 3).  First the off-diagonal
 // 2 x 2 blocks swap between lanes t and t ^ 2, then the elements within them
 // between lanes t and t ^ 1.
+// The half-word moves are byte permutes (v_perm_b32: selector bytes 0-3 pick
+// the second operand's bytes, 4-7 the first's) whose selectors depend on the
+// lane's place only, so the lanes of a quad take no branches.
 __device__ inline uint2 quad_transpose16(uint32_t d0, uint32_t d1, int t) {
   const bool hi = (t & 2) != 0, odd = (t & 1) != 0;
   const uint32_t r = quad_xor2(hi ? d0 : d1);
   const uint32_t e0 = hi ? r : d0, e1 = hi ? d1 : r;
-  const uint32_t q = quad_xor1(odd ? ((e0 & 0xFFFFu) | (e1 << 16)) : ((e0 >> 16) | (e1 & 0xFFFF0000u)));
-  uint2 o;
-  if (odd) {
-    o.x = (q & 0xFFFFu) | (e0 & 0xFFFF0000u);
-    o.y = (q >> 16) | (e1 & 0xFFFF0000u);
-  } else {
-    o.x = (e0 & 0xFFFFu) | (q << 16);
-    o.y = (e1 & 0xFFFFu) | (q & 0xFFFF0000u);
-  }
-  return o;
+  // odd lanes send the low halves of e0, e1, even lanes the high halves
+  const uint32_t q = quad_xor1(__builtin_amdgcn_perm(e1, e0, odd ? 0x05040100u : 0x07060302u));
+  // odd: x = (q.lo, e0.hi), y = (q.hi, e1.hi); even: x = (e0.lo, q.lo), y = (e1.lo, q.hi)
+  return make_uint2(__builtin_amdgcn_perm(e0, q, odd ? 0x07060100u : 0x01000504u),
+                    __builtin_amdgcn_perm(e1, q, odd ? 0x07060302u : 0x03020504u));
 }
 
 // the same for FP32 values
@@ -137,23 +135,56 @@ __device__ inline float4 quad_transpose32(float m0, float m1, float m2, float m3
 
 // Rows mv..mv+3 (one point) of unit n = the lane's column into the
 // unit-blocked X [N/KL][M][KL]; t = n % 4 (the lane's place in its quad).
+// Offsets are unsigned (rows and units are never negative).
 template <class T>
 __device__ inline void st_unit_blocked(T* X, int64_t M, int n, int64_t mv, int t, float v0, float v1, float v2,
                                        float v3) {
-  const int nq = n - t;
+  const uint32_t nq = (uint32_t)(n - t);
   if constexpr (sizeof(T) == 4) {
-    *reinterpret_cast<float4*>(X + ((int64_t)(nq / 4) * M + mv + t) * 4) = quad_transpose32(v0, v1, v2, v3, t);
+    *reinterpret_cast<float4*>(X + ((uint64_t)(nq >> 2) * (uint64_t)M + (uint64_t)mv + t) * 4) =
+        quad_transpose32(v0, v1, v2, v3, t);
   } else {
-    *reinterpret_cast<uint2*>(X + ((int64_t)(nq / 8) * M + mv + t) * 8 + (nq & 7)) =
+    *reinterpret_cast<uint2*>(X + ((uint64_t)(nq >> 3) * (uint64_t)M + (uint64_t)mv + t) * 8 + (nq & 7u)) =
         quad_transpose16(pack2<T>(v0, v1), pack2<T>(v2, v3), t);
   }
+}
+
+// The same from the four values already packed two per dword (16-bit T).
+template <class T>
+__device__ inline void st_unit_blocked16(T* X, int64_t M, int n, int64_t mv, int t, uint2 packed) {
+  const uint32_t nq = (uint32_t)(n - t);
+  *reinterpret_cast<uint2*>(X + ((uint64_t)(nq >> 3) * (uint64_t)M + (uint64_t)mv + t) * 8 + (nq & 7u)) =
+      quad_transpose16(packed.x, packed.y, t);
 }
 
 // Rows mv..mv+3 of unit n into the row-blocked Xb [M/KL][N][KL].
 template <class T>
 __device__ inline void st_row_blocked(T* Xb, int N, int n, int64_t mv, float v0, float v1, float v2, float v3) {
   constexpr int KL = 16 / (int)sizeof(T);
-  st4rows(Xb + ((mv / KL) * N + n) * KL + mv % KL, v0, v1, v2, v3);
+  const uint64_t m = (uint64_t)mv;
+  st4rows(Xb + ((m / KL) * (uint32_t)N + (uint32_t)n) * KL + m % KL, v0, v1, v2, v3);
+}
+
+// The same from the packed values (16-bit T: rows mv..mv+3 are 8 contiguous bytes).
+template <class T>
+__device__ inline void st_row_blocked16(T* Xb, int N, int n, int64_t mv, uint2 packed) {
+  constexpr int KL = 16 / (int)sizeof(T);
+  const uint64_t m = (uint64_t)mv;
+  *reinterpret_cast<uint2*>(Xb + ((m / KL) * (uint32_t)N + (uint32_t)n) * KL + m % KL) = packed;
+}
+
+// Both layouts of one point's four rows, each value converted once.
+template <class T>
+__device__ inline void st_both(T* X, T* Xb, int64_t M, int N, int n, int64_t mv, int t, float v0, float v1, float v2,
+                               float v3) {
+  if constexpr (sizeof(T) == 4) {
+    st_unit_blocked(X, M, n, mv, t, v0, v1, v2, v3);
+    if (Xb) st_row_blocked(Xb, N, n, mv, v0, v1, v2, v3);
+  } else {
+    const uint2 packed = make_uint2(pack2<T>(v0, v1), pack2<T>(v2, v3));
+    st_unit_blocked16(X, M, n, mv, t, packed);
+    if (Xb) st_row_blocked16(Xb, N, n, mv, packed);
+  }
 }
 
 // Sum of V values over the lanes of a lane group, exchanging with the lanes
@@ -343,11 +374,12 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
       v4 zb[TG_BC][PTS];
 #pragma unroll
       for (int pt = 0; pt < PTS; ++pt) {
-        const int64_t p = (m0 + i * R + S::rowmap(g, 4 * pt)) / 4;
+        const uint64_t p = (uint64_t)(m0 + i * R + S::rowmap(g, 4 * pt)) >> 2;
         pv[pt] = (EPI == TG_OUT_BWD || EPI == TG_FIRST_BWD) ? a.pt4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int j = 0; j < TG_BC; ++j) {
-          if constexpr (EPI == TG_ACT_BWD) zb[j][pt] = reinterpret_cast<const v4*>(a.D)[p * N + n0 + j * R + col];
+          if constexpr (EPI == TG_ACT_BWD)
+            zb[j][pt] = reinterpret_cast<const v4*>(a.D)[p * (uint32_t)N + (uint32_t)(n0 + j * R + col)];
         }
       }
       float fo[PTS * 4];  // TG_OUT: (f, grad f) of the lane's points over its columns
@@ -367,10 +399,10 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
             // a = W p + b, p' = sin a, pd'_t = cos(a) ad_t; D = (cos a, sin(a) ad_t)
             float sn, cs;
             sincos_reduced(v0 + bias, sn, cs);
-            st4rows(reinterpret_cast<T*>(a.D) + ((mv / 4) * N + n) * 4, cs, sn * v1, sn * v2, sn * v3);
+            st4rows(reinterpret_cast<T*>(a.D) + (((uint64_t)mv >> 2) * (uint32_t)N + (uint32_t)n) * 4, cs, sn * v1,
+                    sn * v2, sn * v3);
             const float x1 = cs * v1, x2 = cs * v2, x3 = cs * v3;
-            st_unit_blocked(X, M, n, mv, t4, sn, x1, x2, x3);
-            if (Xb) st_row_blocked(Xb, N, n, mv, sn, x1, x2, x3);
+            st_both(X, Xb, M, N, n, mv, t4, sn, x1, x2, x3);
           } else if constexpr (EPI == TG_OUT) {
             // f = sum_u wl_u sin a_u (+ b_out later), grad f_t = sum_u wl_u cos(a_u) ad_t,u
             float sn, cs;
@@ -393,8 +425,7 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
             const float ad1 = wc * sg.y, ad2 = wc * sg.z, ad3 = wc * sg.w;
             csum[j][0] += abar;
             csum[j][1] += fmaf(sg.x, sn, cs * gad);
-            st_unit_blocked(X, M, n, mv, t4, abar, ad1, ad2, ad3);
-            if (Xb) st_row_blocked(Xb, N, n, mv, abar, ad1, ad2, ad3);
+            st_both(X, Xb, M, N, n, mv, t4, abar, ad1, ad2, ad3);
           } else if constexpr (EPI == TG_ACT_BWD) {
             // abar = cos(a) pbar - sin(a) sum_t ad_t pdbar_t, adbar_t = cos(a) pdbar_t
             const v4 d = zb[j][pt];
@@ -402,8 +433,7 @@ __global__ __launch_bounds__(256) void k_tgemm(TgLaunch L) {
             const float abar = fmaf(cs, v0, -fmaf((float)d[3], v3, fmaf((float)d[2], v2, (float)d[1] * v1)));
             const float ad1 = cs * v1, ad2 = cs * v2, ad3 = cs * v3;
             csum[j][0] += abar;
-            st_unit_blocked(X, M, n, mv, t4, abar, ad1, ad2, ad3);
-            if (Xb) st_row_blocked(Xb, N, n, mv, abar, ad1, ad2, ad3);
+            st_both(X, Xb, M, N, n, mv, t4, abar, ad1, ad2, ad3);
           } else {  // TG_FIRST_BWD
             // layer 0: a = W_0 x + b_0, ad_t = W_0[.][t] (coordinate tangents);
             // dW_0[u][i] += abar x_i + adbar_i, db_0[u] += abar
